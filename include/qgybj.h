@@ -1,0 +1,150 @@
+/*
+ * qgybj.h -- C ABI of the MI355X QG-YBJ+ time step.
+ *
+ * Drop-in for the body of the reference's integration loop
+ * (oliasselin/QG-YBJp-model main_waqg.f90:326-467, with its Euler prologue
+ * :197-317).  The reference has no library interface: its loop body is a
+ * sequence of Fortran calls over static arrays.  Each entry point below names
+ * the reference routine(s) it replaces.  A Fortran driver binds these
+ * through ISO_C_BINDING (qg-ybjp-model_amd/fortran/qgybj_c.f90).
+ *
+ * Conventions
+ *  - Spectral arrays are host arrays of interleaved (re, im) FP64 pairs in the
+ *    reference's column-major layout X(iktx, ikty, n3), iktx = n1/2+1,
+ *    ikty = n2, ky order [0..n2/2-1, Nyquist, -n2/2+1..-1] (init.f90:35-41);
+ *    no vertical halo levels.
+ *  - Real (physical) arrays are X(ldx, n2, nz) with ldx >= n1 (the reference
+ *    passes ldx = n1+2, main_waqg.f90:15-110).
+ *  - Every call returns QG_OK (0) or an error code; the library never exits.
+ *    qg_last_error() gives the message of the last failure on a context.
+ *  - One context = one member on one HIP stream of the current device.  Calls
+ *    on one context are serialised by the caller; distinct contexts are
+ *    independent.
+ */
+#ifndef QGYBJ_H
+#define QGYBJ_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  QG_OK = 0,
+  QG_ERR_ARG = 1,       /* invalid argument or unsupported size */
+  QG_ERR_HIP = 2,       /* HIP runtime error */
+  QG_ERR_NOMEM = 3,     /* device allocation failed */
+  QG_ERR_NONFINITE = 4, /* a NaN/Inf appeared in the updated fields */
+  QG_ERR_STATE = 5      /* call not valid in the current state */
+};
+
+/* special-case switches; same meaning as the reference's integer parameters
+ * fixed_flow, no_feedback, passive_scalar, linear, no_dispersion, inviscid
+ * (parameters_test1.f90:22-29).  ybj_plus = 1 is implied. */
+enum {
+  QG_FIXED_FLOW = 1,
+  QG_NO_FEEDBACK = 2,
+  QG_PASSIVE_SCALAR = 4,
+  QG_LINEAR = 8,
+  QG_NO_DISPERSION = 16,
+  QG_INVISCID = 32
+};
+
+/* Runtime form of the reference's compile-time parameters module
+ * (parameters_test1.f90:1-230).  All values are the exact FP64 values the
+ * step uses; the driver forms them (including the REAL(4) literals and powers
+ * of the reference) and passes their bits. */
+typedef struct qg_params {
+  int32_t n1, n2, n3;   /* n1 == n2, a power of two in [16, 512]; n3 >= 2   */
+  int32_t flags;        /* QG_* switches                                     */
+  double dt;            /* delt                                              */
+  double gamma;         /* Robert-Asselin coefficient                        */
+  double Ro, Bu, W2F;   /* Rossby, (inverse) Burger, (Uw/U)^2                */
+  double nuz;           /* vertical viscosity of q (derivatives.f90:1877)    */
+  const double* ifac_q; /* (iktx, ikty): int_factor   (main_waqg.f90:371)    */
+  const double* ifac_w; /* (iktx, ikty): int_factor_w (main_waqg.f90:372)    */
+  const double* N2_u;   /* n3: N^2 at z_u(k) = k dz       (init.f90:232)     */
+  const double* N2_s;   /* n3: N^2 at z_s(k) = (k-1/2) dz (init.f90:233)     */
+} qg_params;
+
+typedef struct qg_ctx qg_ctx;
+
+/* field ids for qg_device_field */
+enum { QG_PSI = 0, QG_Q, QG_QO, QG_BR, QG_BI, QG_BRO, QG_BIO, QG_AR, QG_AI, QG_NFIELDS };
+
+/* Set-up and tear-down.  Replaces initialize_fftw (fft.f90:32-52),
+ * init_arrays (init.f90:17-171) and the coefficient part of init_base_state
+ * (init.f90:177-264): allocates the member's device state, twiddles, masks,
+ * integrating-factor exponentials and vertical operator. */
+int qg_create(const qg_params* p, qg_ctx** out);
+int qg_destroy(qg_ctx* ctx);
+const char* qg_last_error(const qg_ctx* ctx);
+
+/* Load a spectral state (host arrays, see conventions).  NULL arrays load
+ * zeros.  iter = 0 makes the next step the Euler first step
+ * (main_waqg.f90:197-317); iter > 0 continues with leapfrog steps.
+ * Replaces the reference's start-up assignments main_waqg.f90:125-151. */
+int qg_set_state(qg_ctx* ctx, const double* psik, const double* qk, const double* qok, const double* BRk,
+                 const double* BIk, const double* BRok, const double* BIok, const double* ARk, const double* AIk,
+                 int32_t iter);
+
+/* Start from physical initial conditions: un-dealiased r2c of psi, BR, BI
+ * (ncread_psi IO_ncf.f90:276-287, ncread_la :353-366), q = init_q(psi) or 0
+ * under QG_FIXED_FLOW (main_waqg.f90:142-146), A = 0, old fields 0, iter = 0. */
+int qg_set_state_physical(qg_ctx* ctx, const double* psir, const double* BRr, const double* BIr, int32_t ldx);
+
+/* Copy the state out (NULL arrays skipped).  C = A_z as A_solver_ybj_plus
+ * forms it (elliptic.f90:437-450).  Synchronises the context's stream. */
+int qg_get_state(qg_ctx* ctx, double* psik, double* qk, double* qok, double* BRk, double* BIk, double* BRok,
+                 double* BIok, double* ARk, double* AIk, double* CRk, double* CIk);
+
+/* Advance nsteps time steps: the loop body main_waqg.f90:326-467 (Euler
+ * prologue :197-317 when iter = 0): nonlinear terms (convol_waqg,
+ * refraction_waqg), the fused update + Robert filter, compute_qw, psi_solver,
+ * A_solver_ybj_plus, compute_velo.  Returns QG_ERR_NONFINITE if a NaN/Inf
+ * appeared. */
+int qg_step(qg_ctx* ctx, int32_t nsteps);
+int qg_iter(const qg_ctx* ctx, int32_t* iter);
+
+/* FP64 diagnostics of the current state, nondimensional:
+ * out[0] KE and out[1] PE (diag_zentrum ks / ps_quad, diagnostics.f90:127-158),
+ * out[2] WKE, out[3] WPE, out[4] WCE (wave_energy k_p, p_p, c_p,
+ * diagnostics.f90:683-712, without the Uw_scale^2 factor),
+ * out[5] enstrophy 0.5<zeta^2> = sum kh2^2 |psi|^2; all averaged over z. */
+int qg_diag(qg_ctx* ctx, double out[6]);
+
+/* init_q (init.f90:591-652) on host arrays: q = T(kh2) psi / dz^2 at L = 1. */
+int qg_init_q(qg_ctx* ctx, const double* psik, double* qk);
+
+/* fft_r2c / fft_c2r (fft.f90:55-83) on host arrays of nz <= n3 levels.
+ * r2c is unmasked and scaled by 1/(n1 n2); c2r applies realit and is
+ * unnormalised.  Unlike the reference, the c2r input is not modified. */
+int qg_fft_r2c(qg_ctx* ctx, const double* r, int32_t ldx, double* k, int32_t nz);
+int qg_fft_c2r(qg_ctx* ctx, const double* k, double* r, int32_t ldx, int32_t nz);
+
+/* Device-resident access: device address of a state field (complex FP64,
+ * layout as above) and the context's HIP stream (hipStream_t). */
+int qg_device_field(qg_ctx* ctx, int32_t which, void** dptr);
+int qg_stream(qg_ctx* ctx, void** stream);
+int qg_sync(qg_ctx* ctx);
+
+/* Per-kernel timing with HIP events on the context's stream: enable, then
+ * read accumulated milliseconds and launch counts per kernel id
+ * (0 col_inv, 1 row_nl, 2 col_upd, 3 row_qw, 4 col_qw, 5 solve, 6 aux).
+ * Reading resets the accumulators. */
+int qg_timing(qg_ctx* ctx, int32_t enable);
+int qg_kernel_times(qg_ctx* ctx, double* ms, int32_t* counts, int32_t n);
+const char* qg_kernel_name(int32_t id);
+
+/* Host-only helpers (no GPU needed): the dealiasing mask L(iktx, ikty) as the
+ * library evaluates it, and the box geometry {kmax, KX, nlo, nhi}. */
+int qg_mask(int32_t n1, int32_t n2, int32_t* L);
+int qg_box(int32_t n1, int32_t n2, int32_t out[4]);
+const char* qg_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QGYBJ_H */

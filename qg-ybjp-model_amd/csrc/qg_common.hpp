@@ -1,0 +1,100 @@
+// Internal declarations shared by the kernels (qg_kernels.hip) and the C-ABI
+// layer (qg_api.cpp).  Not part of the public interface (include/qgybj.h).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace qg {
+
+// Spectral-grid geometry of one member.  Spectral fields use the reference's
+// Fortran layout (iktx, ikty, n3), kx fastest (main_waqg.f90:15-59).  The hot
+// loop only touches the dealiasing box: kx < KX and rows iky < nlo or
+// iky >= n2 - nhi, which contains every mode with L = 1 (init.f90:44-80).
+struct Geo {
+  int n1, n2, n3;
+  int iktx, ikty;
+  int kmax;    // largest |k| component with L = 1
+  int KX;      // kmax + 1 columns kept
+  int nlo;     // rows 0..kmax     (ky = 0..kmax)
+  int nhi;     // rows n2-kmax..n2-1 (ky = -kmax..-1)
+  int nbox;    // nlo + nhi
+  int kh2max;  // L = 1 requires kx^2 + ky^2 <= kh2max
+};
+
+// ky label of row iky: 0..n2/2-1, the Nyquist row relabelled 0, then
+// -n2/2+1..-1 (init.f90:35-41)
+__host__ __device__ inline int ky_of(int iky, int n2) {
+  return iky < n2 / 2 ? iky : (iky == n2 / 2 ? 0 : iky - n2);
+}
+
+// Dealiasing mask L (init.f90:44-80): zero on the Nyquist row, on kx = 0 with
+// ky < 0, and outside the radius int(n1/3+0.5)-0.5 (an integer bound on kh2).
+__host__ __device__ inline bool lmask(int kx, int iky, const Geo& g) {
+  if (iky == g.n2 / 2) return false;
+  const int ky = ky_of(iky, g.n2);
+  if (kx == 0 && ky < 0) return false;
+  return kx * kx + ky * ky <= g.kh2max;
+}
+
+__host__ __device__ inline bool in_box_row(int iky, const Geo& g) {
+  return iky < g.nlo || iky >= g.n2 - g.nhi;
+}
+__host__ __device__ inline int box_row(int ii, const Geo& g) {
+  return ii < g.nlo ? ii : ii + (g.n2 - g.nlo - g.nhi);
+}
+
+Geo make_geo(int n1, int n2, int n3);
+
+// flag bits, same values as include/qgybj.h
+enum : int {
+  F_FIXED_FLOW = 1, F_NO_FEEDBACK = 2, F_PASSIVE = 4, F_LINEAR = 8, F_NO_DISPERSION = 16, F_INVISCID = 32
+};
+
+// Device-side view of one member: state, work buffers and tables.
+struct Dev {
+  Geo g;
+  int flags;
+  double dt, gamma, Ro, Bu, W2F, nuz, dz;
+  // state (full layout, n3 levels)
+  double2 *psi, *q, *qo, *BR, *BI, *BRo, *BIo, *AR, *AI;
+  // work: q* (and temporaries), dq, intermediates I0..I5 (pitch iktx, n2 rows/level)
+  double2 *qs, *dq;
+  double2* I[6];
+  double* scratch;  // Thomas elimination diagonals, 3 * n3 * ncols * 2
+  // tables
+  const double2* tw;       // TWN twiddles
+  const double *e1q, *e2q, *e1w, *e2w;  // exp(-f), exp(-2f) per (iky, kx)
+  const double *dl, *du, *dsum;         // vertical operator, n3 each
+  const double *a_ell, *N2u, *N2s;      // n3 each
+  double* diag_part;                    // per-block partial sums
+  int* flag;                            // non-finite detector
+};
+
+// kernel ids for per-kernel timing
+enum KernelId : int {
+  K_COL_INV = 0, K_ROW_NL, K_COL_UPD, K_ROW_QW, K_COL_QW, K_SOLVE, K_AUX, K_NKERNELS
+};
+const char* kernel_name(int id);
+
+// ---- launchers (qg_kernels.hip); all enqueue on `st` and return hipError_t
+// column inverse transform of one spectral field into an intermediate
+// mul: 0 -> 1, 1 -> i*ky, 2 -> -ky^2; W columns; box rows only if box
+hipError_t launch_col_inv(const Dev& d, hipStream_t st, const double2* F, double2* out, int W, int box, int mul,
+                          int mask, int nz);
+hipError_t launch_row_nl(const Dev& d, hipStream_t st, int Win);
+hipError_t launch_col_upd(const Dev& d, hipStream_t st, int euler, int has_dq);
+hipError_t launch_row_qw(const Dev& d, hipStream_t st);
+hipError_t launch_col_qw(const Dev& d, hipStream_t st);
+hipError_t launch_solve(const Dev& d, hipStream_t st, int f_lo, int f_hi, const double2* rhs_psi);
+hipError_t launch_dq(const Dev& d, hipStream_t st);
+hipError_t launch_clear_outside(const Dev& d, hipStream_t st, double2* f, int all);
+hipError_t launch_init_q(const Dev& d, hipStream_t st, const double2* psi, double2* q);
+hipError_t launch_cz(const Dev& d, hipStream_t st, const double2* A, double2* C);
+hipError_t launch_diag(const Dev& d, hipStream_t st, int* nblocks);
+hipError_t launch_r2c(const Dev& d, hipStream_t st, const double* r, int ld, double2* k, int nz);
+hipError_t launch_c2r(const Dev& d, hipStream_t st, const double2* k, double* r, int ld, int nz);
+int diag_blocks(const Geo& g);
+bool supported_n(int n);
+
+}  // namespace qg

@@ -1,0 +1,802 @@
+// HIP kernels of the QG-YBJ+ time step for gfx950 (FP64 throughout).
+//
+// Data flow of one leapfrog step (spectral fields in the (iktx, ikty, n3)
+// layout, intermediates I0..I5 in the half-transformed (kx, y) layout):
+//
+//   col_inv  : psi^n            -> I1 = Fy^-1[L psi], I3 = Fy^-1[L i ky psi],
+//                                  I5 = Fy^-1[L (-ky^2) psi]
+//   row_nl   : I0..I5           -> u, v, zeta, q, BR, BI rows (c2r in x),
+//                                  the 8 products, r2c in x, pre-combined
+//                                  flux tendencies -> I0..I5
+//   col_upd  : I0..I5 + state   -> r2c in y, leapfrog + integrating factor +
+//                                  Robert filter in place, then the inverse
+//                                  y transforms of the new q, BR, BI (and of
+//                                  i ky BR, i ky BI) -> I0 (q), I2,I3 (BR),
+//                                  I4,I5 (BI)
+//   row_qw   : I2..I5           -> J(BR,BI) and |B|^2 rows -> I3, I5
+//   col_qw   : I3, I5, q^{n+1}  -> q* = q - q^w
+//   solve    : q*, BR, BI       -> psi, AR, AI (Thomas sweep in z per column)
+//
+// The reference computes the same quantities with 24 per-level 2D FFTW
+// transforms, 8 MPI transposes and 6 DGTSV calls per column per step
+// (main_waqg.f90:326-467, derivatives.f90:607-1032, elliptic.f90:145-460).
+#include "qg_common.hpp"
+#include "qg_fft.hpp"
+
+namespace qg {
+
+static __device__ __forceinline__ double2 zc() { return make_double2(0.0, 0.0); }
+
+// Two real half-spectra A, B (entry k of each) packed into one complex
+// sequence Z whose inverse DFT is a + i b (a, b the real c2r of A and B).
+// The imaginary parts of the k = 0 and k = N/2 entries are ignored, which is
+// the c2r convention of FFTW and pocketfft.
+template <int N>
+__device__ __forceinline__ void put_pair(double2* lds, int S, int k, int s, double2 A, double2 B) {
+  if (k == 0 || k == N / 2) {
+    lds[k * S + s] = make_double2(A.x, B.x);
+  } else {
+    lds[k * S + s] = make_double2(A.x - B.y, A.y + B.x);
+    lds[(N - k) * S + s] = make_double2(A.x + B.y, B.x - A.y);
+  }
+}
+
+// Split the DFT Z of z = a + i b (a, b real) into A = DFT(a), B = DFT(b) at k.
+template <int N>
+__device__ __forceinline__ void split_pair(const double2* lds, int S, int k, int s, double2& A, double2& B) {
+  const double2 Z = lds[k * S + s];
+  const double2 Zm = cconj(lds[((N - k) & (N - 1)) * S + s]);
+  A = make_double2((Z.x + Zm.x) * 0.5, (Z.y + Zm.y) * 0.5);
+  const double2 dd = csub(Z, Zm);
+  B = make_double2(dd.y * 0.5, -dd.x * 0.5);
+}
+
+__device__ __forceinline__ double2 mul_iky(double2 v, double ky) { return make_double2(-ky * v.y, ky * v.x); }
+
+// ---------------------------------------------------------------------------
+// configuration per transform length (columns per tile, rows per block)
+// ---------------------------------------------------------------------------
+template <int N>
+struct Cfg;
+template <> struct Cfg<16>  { static constexpr int TXI = 16, TXU = 16, PR = 16, PRC = 16; };
+template <> struct Cfg<32>  { static constexpr int TXI = 16, TXU = 16, PR = 8,  PRC = 16; };
+template <> struct Cfg<64>  { static constexpr int TXI = 16, TXU = 16, PR = 8,  PRC = 16; };
+template <> struct Cfg<128> { static constexpr int TXI = 16, TXU = 8,  PR = 4,  PRC = 16; };
+template <> struct Cfg<256> { static constexpr int TXI = 16, TXU = 8,  PR = 4,  PRC = 16; };
+template <> struct Cfg<512> { static constexpr int TXI = 8,  TXU = 4,  PR = 2,  PRC = 16; };
+constexpr int NT = 256;
+
+// ---------------------------------------------------------------------------
+// col_inv: inverse DFT in y of one spectral field, with an optional spectral
+// multiplier, the mask L and the reference's realit fill of (0,-ky)
+// (fft.f90:71-107).  Output: intermediate (kx, y) rows, columns kx < W.
+// ---------------------------------------------------------------------------
+template <int N, int TX>
+__global__ __launch_bounds__(NT) void k_col_inv(Geo g, const double2* __restrict__ tw, const double2* __restrict__ F,
+                                                 double2* __restrict__ out, int W, int box, int mul, int mask) {
+  __shared__ double2 lds[N * TX];
+  const int kx0 = blockIdx.x * TX, z = blockIdx.y;
+  const double2* Fz = F + (size_t)z * g.ikty * g.iktx;
+  for (int e = threadIdx.x; e < N * TX; e += NT) {
+    const int i = e / TX, c = e % TX, kx = kx0 + c;
+    double2 v = zc();
+    if (kx < W && (!box || in_box_row(i, g)) && (!mask || lmask(kx, i, g))) {
+      v = Fz[(size_t)i * g.iktx + kx];
+      const double ky = (double)ky_of(i, g.n2);
+      if (mul == 1) v = mul_iky(v, ky);
+      else if (mul == 2) v = cscale(v, -(ky * ky));
+    }
+    lds[i * TX + c] = v;
+  }
+  __syncthreads();
+  if (kx0 == 0) {
+    for (int j = 1 + (int)threadIdx.x; j < N / 2; j += NT) lds[(N - j) * TX] = cconj(lds[j * TX]);
+    __syncthreads();
+  }
+  fft_lds<N, true, TX, NT>(lds, tw);
+  double2* oz = out + (size_t)z * g.n2 * g.iktx;
+  for (int e = threadIdx.x; e < N * TX; e += NT) {
+    const int i = e / TX, c = e % TX, kx = kx0 + c;
+    if (kx < W) oz[(size_t)i * g.iktx + kx] = lds[i * TX + c];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// row_nl: per (z, y) row, c2r in x of u, v, zeta, q, BR, BI, the products of
+// convol_waqg / refraction_waqg (derivatives.f90:661-674, 709-722, 756-769,
+// 862-875), r2c in x and the pre-combination of the flux divergences:
+//   I0 = i kx F(uq)                   I1 = F(vq)
+//   I2 = i kx F(uBR) - F(zeta BI)/2   I3 = F(vBR)
+//   I4 = i kx F(uBI) + F(zeta BR)/2   I5 = F(vBI)
+// so that after the y transform  n_q = I0 + i ky I1,
+// n_BR - r_BI/2 = I2 + i ky I3 and n_BI + r_BR/2 = I4 + i ky I5.
+// ---------------------------------------------------------------------------
+template <int N, int P>
+__global__ __launch_bounds__(NT) void k_row_nl(Dev d, int Win, double scale) {
+  constexpr int S1 = 3 * P, S2 = 4 * P, H = N / 2 + 1;
+  constexpr int PPT = (P * N + NT - 1) / NT;
+  __shared__ double2 lds[N * S2];
+  const Geo& g = d.g;
+  const size_t row0 = (size_t)blockIdx.x * P;
+  double2* const* I = d.I;
+  for (int e = threadIdx.x; e < P * H; e += NT) {
+    const int p = e / H, k = e % H;
+    const size_t off = (row0 + p) * g.iktx + k;
+    double2 ps = zc(), py = zc(), pyy = zc(), pq = zc(), pbr = zc(), pbi = zc();
+    if (k < Win) {
+      ps = I[1][off]; py = I[3][off]; pyy = I[5][off];
+      pq = I[0][off]; pbr = I[2][off]; pbi = I[4][off];
+    }
+    const double kk = (double)k, k2 = kk * kk;
+    const double2 U = make_double2(-py.x, -py.y);
+    const double2 V = make_double2(-kk * ps.y, kk * ps.x);
+    const double2 Z = make_double2(-k2 * ps.x + pyy.x, -k2 * ps.y + pyy.y);
+    put_pair<N>(lds, S1, k, 3 * p + 0, U, V);
+    put_pair<N>(lds, S1, k, 3 * p + 1, Z, pq);
+    put_pair<N>(lds, S1, k, 3 * p + 2, pbr, pbi);
+  }
+  __syncthreads();
+  fft_lds<N, true, S1, NT>(lds, d.tw);
+  double2 pr[PPT][4];
+#pragma unroll
+  for (int q = 0; q < PPT; ++q) {
+    const int e = threadIdx.x + q * NT;
+    if (e < P * N) {
+      const int p = e % P, n = e / P;
+      const double2 a = lds[n * S1 + 3 * p], b = lds[n * S1 + 3 * p + 1], c = lds[n * S1 + 3 * p + 2];
+      const double u = a.x, v = a.y, zeta = b.x, qv = b.y, br = c.x, bi = c.y;
+      pr[q][0] = make_double2(u * qv, v * qv);
+      pr[q][1] = make_double2(u * br, v * br);
+      pr[q][2] = make_double2(u * bi, v * bi);
+      pr[q][3] = make_double2(zeta * br, zeta * bi);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < PPT; ++q) {
+    const int e = threadIdx.x + q * NT;
+    if (e < P * N) {
+      const int p = e % P, n = e / P;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) lds[n * S2 + 4 * p + c] = pr[q][c];
+    }
+  }
+  __syncthreads();
+  fft_lds<N, false, S2, NT>(lds, d.tw);
+  const bool lin = (d.flags & F_LINEAR) != 0, pas = (d.flags & F_PASSIVE) != 0;
+  for (int e = threadIdx.x; e < P * g.KX; e += NT) {
+    const int p = e / g.KX, k = e % g.KX;
+    const size_t off = (row0 + p) * g.iktx + k;
+    double2 Fuq, Fvq, FuR, FvR, FuI, FvI, FzR, FzI;
+    split_pair<N>(lds, S2, k, 4 * p + 0, Fuq, Fvq);
+    split_pair<N>(lds, S2, k, 4 * p + 1, FuR, FvR);
+    split_pair<N>(lds, S2, k, 4 * p + 2, FuI, FvI);
+    split_pair<N>(lds, S2, k, 4 * p + 3, FzR, FzI);
+    const double kk = (double)k;
+    double2 w0 = mul_iky(Fuq, kk), w1 = Fvq;
+    double2 w2 = mul_iky(FuR, kk), w3 = FvR;
+    double2 w4v = mul_iky(FuI, kk), w5 = FvI;
+    if (lin) { w0 = zc(); w1 = zc(); w2 = zc(); w3 = zc(); w4v = zc(); w5 = zc(); }
+    if (!pas) {
+      w2 = make_double2(w2.x - 0.5 * FzI.x, w2.y - 0.5 * FzI.y);
+      w4v = make_double2(w4v.x + 0.5 * FzR.x, w4v.y + 0.5 * FzR.y);
+    }
+    I[0][off] = cscale(w0, scale); I[1][off] = cscale(w1, scale);
+    I[2][off] = cscale(w2, scale); I[3][off] = cscale(w3, scale);
+    I[4][off] = cscale(w4v, scale); I[5][off] = cscale(w5, scale);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// col_upd: r2c in y of the pre-combined tendencies of one field, the time
+// update (Euler: main_waqg.f90:251-253; leapfrog + Robert-Asselin filter:
+// :375-377, :400-402, :413-415) in place, then the inverse y transforms of
+// the new field for the next step's products and for the feedback term
+// (derivatives.f90:938-956).  blockIdx.z: 0 = q, 1 = BR, 2 = BI.
+// ---------------------------------------------------------------------------
+template <int N, int TX>
+__global__ __launch_bounds__(NT) void k_col_upd(Dev d, int euler, int has_dq) {
+  constexpr int S = 2 * TX;
+  constexpr int MAXE = (N * TX + NT - 1) / NT;
+  __shared__ double2 lds[N * S];
+  const Geo& g = d.g;
+  const int kx0 = blockIdx.x * TX, z = blockIdx.y, f = blockIdx.z;
+  double2* W1 = d.I[2 * f];
+  double2* W2 = d.I[2 * f + 1];
+  const size_t pz = (size_t)z * g.n2 * g.iktx;
+  for (int e = threadIdx.x; e < N * S; e += NT) {
+    const int i = e / S, s = e % S, c = s % TX, kx = kx0 + c;
+    double2 v = zc();
+    if (kx < g.KX) v = (s < TX ? W1 : W2)[pz + (size_t)i * g.iktx + kx];
+    lds[i * S + s] = v;
+  }
+  __syncthreads();
+  fft_lds<N, false, S, NT>(lds, d.tw);
+
+  double2* X = f == 0 ? d.q : (f == 1 ? d.BR : d.BI);
+  double2* Xo = f == 0 ? d.qo : (f == 1 ? d.BRo : d.BIo);
+  const double2* Av = f == 1 ? d.AI : d.AR;
+  const bool useA = f > 0 && !(d.flags & (F_NO_DISPERSION | F_PASSIVE));
+  const double* e1t = f == 0 ? d.e1q : d.e1w;
+  const double* e2t = f == 0 ? d.e2q : d.e2w;
+  const double cd = 0.5 / (d.Bu * d.Ro);
+  const double sgn = f == 1 ? 1.0 : -1.0;  // +c kh2 AI for BR, -c kh2 AR for BI
+  const size_t sz = (size_t)z * g.ikty * g.iktx;
+  double2 xn[MAXE];
+  bool bad = false;
+#pragma unroll
+  for (int q = 0; q < MAXE; ++q) {
+    const int e = threadIdx.x + q * NT;
+    xn[q] = zc();
+    if (e < g.nbox * TX) {
+      const int ii = e / TX, c = e % TX, kx = kx0 + c, iky = box_row(ii, g);
+      if (kx < g.KX) {
+        const double ky = (double)ky_of(iky, g.n2);
+        const double2 F1 = lds[iky * S + c], F2 = lds[iky * S + TX + c];
+        const double2 t = make_double2(F1.x - ky * F2.y, F1.y + ky * F2.x);
+        const size_t kidx = (size_t)iky * g.iktx + kx;
+        const size_t idx = sz + kidx;
+        double2 xnew = zc(), xonew = zc();
+        if (lmask(kx, iky, g)) {
+          const double2 x = X[idx];
+          const double e1 = e1t[kidx];
+          const double kh2 = (double)kx * (double)kx + ky * ky;
+          double2 dqv = zc();
+          if (f == 0 && has_dq) dqv = d.dq[idx];
+          double2 av = zc();
+          if (useA) av = Av[idx];
+          if (euler) {
+            if (f == 0) {
+              xnew = make_double2(((x.x - d.dt * t.x) + d.dt * dqv.x) * e1, ((x.y - d.dt * t.y) + d.dt * dqv.y) * e1);
+            } else {
+              const double ck = (d.dt * cd) * kh2;
+              xnew = make_double2(((x.x - d.dt * t.x) - sgn * (ck * av.x)) * e1,
+                                  ((x.y - d.dt * t.y) - sgn * (ck * av.y)) * e1);
+            }
+          } else {
+            const double2 xo = Xo[idx];
+            const double e2 = e2t[kidx];
+            const double tdt = 2.0 * d.dt;
+            if (f == 0) {
+              xnew = make_double2((xo.x * e2 - (tdt * t.x) * e1) + (tdt * dqv.x) * e2,
+                                  (xo.y * e2 - (tdt * t.y) * e1) + (tdt * dqv.y) * e2);
+            } else {
+              const double ck = cd * kh2;
+              xnew = make_double2(xo.x * e2 - (tdt * (t.x + sgn * (ck * av.x))) * e1,
+                                  xo.y * e2 - (tdt * (t.y + sgn * (ck * av.y))) * e1);
+            }
+            xonew = make_double2(x.x + d.gamma * ((xo.x - 2.0 * x.x) + xnew.x),
+                                 x.y + d.gamma * ((xo.y - 2.0 * x.y) + xnew.y));
+          }
+        }
+        X[idx] = xnew;
+        if (!euler) Xo[idx] = xonew;
+        bad |= !(isfinite(xnew.x) && isfinite(xnew.y));
+        xn[q] = xnew;
+      }
+    }
+  }
+  if (bad) atomicOr(d.flag, 1);
+  __syncthreads();
+  // stage the new field (and i ky times it) for the inverse y transforms
+#pragma unroll
+  for (int q = 0; q < MAXE; ++q) {
+    const int e = threadIdx.x + q * NT;
+    if (e < g.nbox * TX) {
+      const int ii = e / TX, c = e % TX, iky = box_row(ii, g);
+      const double ky = (double)ky_of(iky, g.n2);
+      lds[iky * S + c] = xn[q];
+      lds[iky * S + TX + c] = mul_iky(xn[q], ky);
+    }
+  }
+  for (int e = threadIdx.x; e < (N - g.nbox) * TX; e += NT) {
+    const int iky = g.nlo + e / TX, c = e % TX;
+    lds[iky * S + c] = zc();
+    lds[iky * S + TX + c] = zc();
+  }
+  __syncthreads();
+  if (kx0 == 0) {
+    for (int j = 1 + (int)threadIdx.x; j < N / 2; j += NT) {
+      lds[(N - j) * S] = cconj(lds[j * S]);
+      lds[(N - j) * S + TX] = cconj(lds[j * S + TX]);
+    }
+    __syncthreads();
+  }
+  fft_lds<N, true, S, NT>(lds, d.tw);
+  for (int e = threadIdx.x; e < N * TX; e += NT) {
+    const int i = e / TX, c = e % TX, kx = kx0 + c;
+    if (kx < g.KX) {
+      W1[pz + (size_t)i * g.iktx + kx] = lds[i * S + c];
+      if (f > 0) W2[pz + (size_t)i * g.iktx + kx] = lds[i * S + TX + c];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// row_qw: c2r in x of BR, BI, their x and y derivatives, the feedback
+// products J = BRy BIx - BRx BIy and |B|^2 (derivatives.f90:967-1000), r2c in
+// x; I3 = F(J) - kx^2/4 F(|B|^2), I5 = F(|B|^2)
+// ---------------------------------------------------------------------------
+template <int N, int P>
+__global__ __launch_bounds__(NT) void k_row_qw(Dev d, double scale) {
+  constexpr int S1 = 3 * P, S2 = P, H = N / 2 + 1;
+  constexpr int PPT = (P * N + NT - 1) / NT;
+  __shared__ double2 lds[N * S1];
+  const Geo& g = d.g;
+  const size_t row0 = (size_t)blockIdx.x * P;
+  double2* const* I = d.I;
+  for (int e = threadIdx.x; e < P * H; e += NT) {
+    const int p = e / H, k = e % H;
+    const size_t off = (row0 + p) * g.iktx + k;
+    double2 br = zc(), bry = zc(), bi = zc(), biy = zc();
+    if (k < g.KX) { br = I[2][off]; bry = I[3][off]; bi = I[4][off]; biy = I[5][off]; }
+    const double kk = (double)k;
+    put_pair<N>(lds, S1, k, 3 * p + 0, br, bi);
+    put_pair<N>(lds, S1, k, 3 * p + 1, mul_iky(br, kk), mul_iky(bi, kk));
+    put_pair<N>(lds, S1, k, 3 * p + 2, bry, biy);
+  }
+  __syncthreads();
+  fft_lds<N, true, S1, NT>(lds, d.tw);
+  double2 pr[PPT];
+#pragma unroll
+  for (int q = 0; q < PPT; ++q) {
+    const int e = threadIdx.x + q * NT;
+    if (e < P * N) {
+      const int p = e % P, n = e / P;
+      const double2 a = lds[n * S1 + 3 * p], b = lds[n * S1 + 3 * p + 1], c = lds[n * S1 + 3 * p + 2];
+      // a = (BR, BI), b = (BRx, BIx), c = (BRy, BIy)
+      pr[q] = make_double2(c.x * b.y - b.x * c.y, a.x * a.x + a.y * a.y);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < PPT; ++q) {
+    const int e = threadIdx.x + q * NT;
+    if (e < P * N) {
+      const int p = e % P, n = e / P;
+      lds[n * S2 + p] = pr[q];
+    }
+  }
+  __syncthreads();
+  fft_lds<N, false, S2, NT>(lds, d.tw);
+  for (int e = threadIdx.x; e < P * g.KX; e += NT) {
+    const int p = e / g.KX, k = e % g.KX;
+    const size_t off = (row0 + p) * g.iktx + k;
+    double2 FJ, FM;
+    split_pair<N>(lds, S2, k, p, FJ, FM);
+    const double c4 = 0.25 * (double)k * (double)k;
+    I[3][off] = cscale(make_double2(FJ.x - c4 * FM.x, FJ.y - c4 * FM.y), scale);
+    I[5][off] = cscale(FM, scale);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// col_qw: r2c in y of the feedback rows, q^w = Ro W2F L (F(J) - kh2/4 F(|B|^2))
+// (derivatives.f90:1010-1027) and q* = q - q^w where L = 1
+// (main_waqg.f90:430-439)
+// ---------------------------------------------------------------------------
+template <int N, int TX>
+__global__ __launch_bounds__(NT) void k_col_qw(Dev d) {
+  constexpr int S = 2 * TX;
+  __shared__ double2 lds[N * S];
+  const Geo& g = d.g;
+  const int kx0 = blockIdx.x * TX, z = blockIdx.y;
+  const size_t pz = (size_t)z * g.n2 * g.iktx;
+  for (int e = threadIdx.x; e < N * S; e += NT) {
+    const int i = e / S, s = e % S, c = s % TX, kx = kx0 + c;
+    double2 v = zc();
+    if (kx < g.KX) v = (s < TX ? d.I[3] : d.I[5])[pz + (size_t)i * g.iktx + kx];
+    lds[i * S + s] = v;
+  }
+  __syncthreads();
+  fft_lds<N, false, S, NT>(lds, d.tw);
+  const size_t sz = (size_t)z * g.ikty * g.iktx;
+  for (int e = threadIdx.x; e < g.nbox * TX; e += NT) {
+    const int ii = e / TX, c = e % TX, kx = kx0 + c, iky = box_row(ii, g);
+    if (kx < g.KX) {
+      const size_t idx = sz + (size_t)iky * g.iktx + kx;
+      double2 out = zc();
+      if (lmask(kx, iky, g)) {
+        const double ky = (double)ky_of(iky, g.n2);
+        const double2 F1 = lds[iky * S + c], F2 = lds[iky * S + TX + c];
+        const double c4 = 0.25 * (ky * ky);
+        const double2 qw = make_double2(((F1.x - c4 * F2.x) * d.Ro) * d.W2F, ((F1.y - c4 * F2.y) * d.Ro) * d.W2F);
+        const double2 qv = d.q[idx];
+        out = csub(qv, qw);
+      }
+      d.qs[idx] = out;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// solve: T(kh2) x = s * rhs for every box column, Re and Im parts in adjacent
+// lanes (psi_solver elliptic.f90:145-247, A_solver_ybj_plus :346-460).
+// LAPACK DGTSV elimination order without interchanges (the systems are
+// strictly diagonally dominant); the eliminated diagonal is kept in scratch
+// for the back substitution.  blockIdx.y: 0 = psi from q*, 1 = AR from BR,
+// 2 = AI from BI.  Columns with kh2 = 0 or L = 0 get x = 0.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void k_solve(Dev d, int f_lo, const double2* __restrict__ rhs_psi) {
+  const Geo& g = d.g;
+  const int f = f_lo + blockIdx.y;
+  const int ncols = g.KX * g.nbox;
+  const int gid = blockIdx.x * NT + threadIdx.x;
+  const int col = gid >> 1, part = gid & 1;
+  if (col >= ncols) return;
+  const int kx = col % g.KX, iky = box_row(col / g.KX, g);
+  const int ky = ky_of(iky, g.n2);
+  const int kh2 = kx * kx + ky * ky;
+  const double2* rhs = f == 0 ? rhs_psi : (f == 1 ? d.BR : d.BI);
+  double2* out = f == 0 ? d.psi : (f == 1 ? d.AR : d.AI);
+  const double* rin = reinterpret_cast<const double*>(rhs) + part;
+  double* xo = reinterpret_cast<double*>(out) + part;
+  const size_t plane2 = (size_t)2 * g.ikty * g.iktx;
+  const size_t c2 = (size_t)2 * ((size_t)iky * g.iktx + kx);
+  const int n3 = g.n3;
+  if (kh2 == 0 || !lmask(kx, iky, g)) {
+    for (int z = 0; z < n3; ++z) xo[z * plane2 + c2] = 0.0;
+    return;
+  }
+  const double dz = d.dz;
+  const double kt = f == 0 ? ((double)kh2 * dz) * dz : (((double)kh2 * dz) * dz) / 4.0;
+  const double rs = f == 0 ? dz * dz : (dz * dz) * d.Bu;
+  double* dsc = d.scratch + ((size_t)f * n3 * ncols) * 2 + (size_t)col * 2 + part;
+  const size_t dst = (size_t)ncols * 2;
+  double dp = -(d.dsum[0] + kt);
+  double bp = rs * rin[c2];
+  xo[c2] = bp;
+  dsc[0] = dp;
+  for (int z = 1; z < n3; ++z) {
+    const double fact = d.dl[z - 1] / dp;
+    const double dn = -(d.dsum[z] + kt) - fact * d.du[z - 1];
+    const double bn = rs * rin[z * plane2 + c2] - fact * bp;
+    xo[z * plane2 + c2] = bn;
+    dsc[z * dst] = dn;
+    dp = dn;
+    bp = bn;
+  }
+  double x = bp / dp;
+  xo[(size_t)(n3 - 1) * plane2 + c2] = x;
+  for (int z = n3 - 2; z >= 0; --z) {
+    x = (xo[z * plane2 + c2] - d.du[z] * x) / dsc[z * dst];
+    xo[z * plane2 + c2] = x;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// vertical diffusion of the old q (derivatives.f90:1877-1926), box modes
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void k_dq(Dev d) {
+  const Geo& g = d.g;
+  const int per = g.KX * g.nbox;
+  const int e = blockIdx.x * NT + threadIdx.x;
+  if (e >= per * g.n3) return;
+  const int z = e / per, r = e % per, kx = r % g.KX, iky = box_row(r / g.KX, g);
+  const size_t plane = (size_t)g.ikty * g.iktx;
+  const size_t idx = z * plane + (size_t)iky * g.iktx + kx;
+  double2 v = zc();
+  if (lmask(kx, iky, g)) {
+    const double dz2 = d.dz * d.dz;
+    const double2 c = d.qo[idx];
+    if (z == 0) {
+      const double2 u = d.qo[idx + plane];
+      v = make_double2(d.nuz * (u.x - c.x) / dz2, d.nuz * (u.y - c.y) / dz2);
+    } else if (z == g.n3 - 1) {
+      const double2 l = d.qo[idx - plane];
+      v = make_double2(d.nuz * (l.x - c.x) / dz2, d.nuz * (l.y - c.y) / dz2);
+    } else {
+      const double2 u = d.qo[idx + plane], l = d.qo[idx - plane];
+      v = make_double2(d.nuz * ((u.x - 2.0 * c.x) + l.x) / dz2, d.nuz * ((u.y - 2.0 * c.y) + l.y) / dz2);
+    }
+  }
+  d.dq[idx] = v;
+}
+
+// zero every mode outside the box (all = 1: every mode)
+__global__ __launch_bounds__(NT) void k_clear(Geo g, double2* f, int all) {
+  const size_t plane = (size_t)g.ikty * g.iktx;
+  const size_t e = (size_t)blockIdx.x * NT + threadIdx.x;
+  if (e >= plane * g.n3) return;
+  const size_t r = e % plane;
+  const int iky = (int)(r / g.iktx), kx = (int)(r % g.iktx);
+  if (all || kx >= g.KX || !in_box_row(iky, g)) f[e] = zc();
+}
+
+// init_q (init.f90:591-652): q = T(kh2) psi / dz^2 at L = 1 modes
+__global__ __launch_bounds__(NT) void k_init_q(Dev d, const double2* __restrict__ psi, double2* __restrict__ q) {
+  const Geo& g = d.g;
+  const size_t plane = (size_t)g.ikty * g.iktx;
+  const size_t e = (size_t)blockIdx.x * NT + threadIdx.x;
+  if (e >= plane * g.n3) return;
+  const int z = (int)(e / plane);
+  const size_t r = e % plane;
+  const int iky = (int)(r / g.iktx), kx = (int)(r % g.iktx);
+  double2 v = zc();
+  if (lmask(kx, iky, g)) {
+    const int ky = ky_of(iky, g.n2);
+    const double dz = d.dz;
+    const double kt = ((double)(kx * kx + ky * ky) * dz) * dz;
+    const double* a = d.a_ell;
+    const int n3 = g.n3;
+    const double2 c = psi[e];
+    if (z == 0) {
+      const double2 u = psi[e + plane];
+      v = make_double2(a[0] * u.x - (a[0] + kt) * c.x, a[0] * u.y - (a[0] + kt) * c.y);
+    } else if (z == n3 - 1) {
+      const double2 l = psi[e - plane];
+      v = make_double2(-(a[n3 - 2] + kt) * c.x + a[n3 - 2] * l.x, -(a[n3 - 2] + kt) * c.y + a[n3 - 2] * l.y);
+    } else {
+      const double2 u = psi[e + plane], l = psi[e - plane];
+      const double dg = (a[z] + a[z - 1]) + kt;
+      v = make_double2((a[z] * u.x - dg * c.x) + a[z - 1] * l.x, (a[z] * u.y - dg * c.y) + a[z - 1] * l.y);
+    }
+    v = make_double2(v.x / (dz * dz), v.y / (dz * dz));
+  }
+  q[e] = v;
+}
+
+// C = A_z: forward difference, zero at the top (elliptic.f90:437-450)
+__global__ __launch_bounds__(NT) void k_cz(Dev d, const double2* __restrict__ A, double2* __restrict__ C) {
+  const Geo& g = d.g;
+  const size_t plane = (size_t)g.ikty * g.iktx;
+  const size_t e = (size_t)blockIdx.x * NT + threadIdx.x;
+  if (e >= plane * g.n3) return;
+  const int z = (int)(e / plane);
+  double2 v = zc();
+  if (z < g.n3 - 1) {
+    const double2 a1 = A[e + plane], a0 = A[e];
+    v = make_double2((a1.x - a0.x) / d.dz, (a1.y - a0.y) / d.dz);
+  }
+  C[e] = v;
+}
+
+// FP64 diagnostic sums (diagnostics.f90:127-158, :683-712; enstrophy as
+// sum kh2^2 |psi|^2), one partial per block, 6 values:
+// KE, PE, WKE, WPE, WCE, ENS (each still to be divided by n3)
+__global__ __launch_bounds__(NT) void k_diag(Dev d) {
+  const Geo& g = d.g;
+  const int per = g.KX * g.nbox;
+  const int e = blockIdx.x * NT + threadIdx.x;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  if (e < per * g.n3) {
+    const int z = e / per, r = e % per, kx = r % g.KX, iky = box_row(r / g.KX, g);
+    if (lmask(kx, iky, g)) {
+      const size_t plane = (size_t)g.ikty * g.iktx;
+      const size_t idx = z * plane + (size_t)iky * g.iktx + kx;
+      const int ky = ky_of(iky, g.n2);
+      const double kh2 = (double)(kx * kx + ky * ky);
+      const double w = (kx == 0 && iky == 0) ? 0.5 : 1.0;
+      const double2 ps = d.psi[idx];
+      const double a2 = ps.x * ps.x + ps.y * ps.y;
+      const double dz = d.dz;
+      double2 b;
+      if (z == 0) {
+        const double2 u = d.psi[idx + plane];
+        b = make_double2((u.x - ps.x) / (2.0 * dz), (u.y - ps.y) / (2.0 * dz));
+      } else if (z == g.n3 - 1) {
+        const double2 l = d.psi[idx - plane];
+        b = make_double2((ps.x - l.x) / (2.0 * dz), (ps.y - l.y) / (2.0 * dz));
+      } else {
+        const double2 u = d.psi[idx + plane], l = d.psi[idx - plane];
+        b = make_double2((u.x - l.x) / (2.0 * dz), (u.y - l.y) / (2.0 * dz));
+      }
+      const double Bu = d.Bu;
+      const double2 br = d.BR[idx], bi = d.BI[idx], ar = d.AR[idx], ai = d.AI[idx];
+      double2 cr = zc(), ci = zc();
+      if (z < g.n3 - 1) {
+        const double2 ar1 = d.AR[idx + plane], ai1 = d.AI[idx + plane];
+        cr = make_double2((ar1.x - ar.x) / dz, (ar1.y - ar.y) / dz);
+        ci = make_double2((ai1.x - ai.x) / dz, (ai1.y - ai.y) / dz);
+      }
+      acc[0] = w * kh2 * a2;
+      acc[1] = w * (b.x * b.x + b.y * b.y) * (Bu / d.N2s[z]);
+      acc[2] = w * (br.x * br.x + br.y * br.y + bi.x * bi.x + bi.y * bi.y);
+      acc[3] = (0.5 / (d.N2u[z] * Bu)) * kh2 * (cr.x * cr.x + cr.y * cr.y + ci.x * ci.x + ci.y * ci.y);
+      acc[4] = (1.0 / 8.0) * (1.0 / (Bu * Bu)) * kh2 * kh2 * (ar.x * ar.x + ar.y * ar.y + ai.x * ai.x + ai.y * ai.y);
+      acc[5] = kh2 * kh2 * a2;
+    }
+  }
+  __shared__ double red[6][NT];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) red[j][threadIdx.x] = acc[j];
+  __syncthreads();
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+#pragma unroll
+      for (int j = 0; j < 6; ++j) red[j][threadIdx.x] += red[j][threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) d.diag_part[(size_t)blockIdx.x * 6 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// ---------------------------------------------------------------------------
+// raw transforms (fft.f90:55-83) for initial conditions and tests
+// ---------------------------------------------------------------------------
+// r2c in x of pairs of real rows -> half rows (all kx), scaled
+template <int N, int P>
+__global__ __launch_bounds__(NT) void k_row_r2c(Geo g, const double2* __restrict__ tw, const double* __restrict__ r,
+                                                 int ld, double2* __restrict__ out, double scale) {
+  constexpr int S = P / 2, H = N / 2 + 1;
+  __shared__ double2 lds[N * S];
+  const size_t row0 = (size_t)blockIdx.x * P;
+  for (int e = threadIdx.x; e < S * N; e += NT) {
+    const int s = e / N, x = e % N;
+    lds[x * S + s] = make_double2(r[(row0 + 2 * s) * ld + x], r[(row0 + 2 * s + 1) * ld + x]);
+  }
+  __syncthreads();
+  fft_lds<N, false, S, NT>(lds, tw);
+  for (int e = threadIdx.x; e < S * H; e += NT) {
+    const int s = e / H, k = e % H;
+    double2 A, B;
+    split_pair<N>(lds, S, k, s, A, B);
+    out[(row0 + 2 * s) * g.iktx + k] = cscale(A, scale);
+    out[(row0 + 2 * s + 1) * g.iktx + k] = cscale(B, scale);
+  }
+}
+
+// forward DFT in y of half rows -> spectral field (all rows, all kx)
+template <int N, int TX>
+__global__ __launch_bounds__(NT) void k_col_fwd(Geo g, const double2* __restrict__ tw, const double2* __restrict__ in,
+                                                 double2* __restrict__ F) {
+  __shared__ double2 lds[N * TX];
+  const int kx0 = blockIdx.x * TX, z = blockIdx.y;
+  const double2* iz = in + (size_t)z * g.n2 * g.iktx;
+  for (int e = threadIdx.x; e < N * TX; e += NT) {
+    const int i = e / TX, c = e % TX, kx = kx0 + c;
+    lds[i * TX + c] = kx < g.iktx ? iz[(size_t)i * g.iktx + kx] : zc();
+  }
+  __syncthreads();
+  fft_lds<N, false, TX, NT>(lds, tw);
+  double2* Fz = F + (size_t)z * g.ikty * g.iktx;
+  for (int e = threadIdx.x; e < N * TX; e += NT) {
+    const int i = e / TX, c = e % TX, kx = kx0 + c;
+    if (kx < g.iktx) Fz[(size_t)i * g.iktx + kx] = lds[i * TX + c];
+  }
+}
+
+// c2r in x of pairs of half rows (all kx) -> real rows
+template <int N, int P>
+__global__ __launch_bounds__(NT) void k_row_c2r(Geo g, const double2* __restrict__ tw, const double2* __restrict__ in,
+                                                 double* __restrict__ r, int ld) {
+  constexpr int S = P / 2, H = N / 2 + 1;
+  __shared__ double2 lds[N * S];
+  const size_t row0 = (size_t)blockIdx.x * P;
+  for (int e = threadIdx.x; e < S * H; e += NT) {
+    const int s = e / H, k = e % H;
+    put_pair<N>(lds, S, k, s, in[(row0 + 2 * s) * g.iktx + k], in[(row0 + 2 * s + 1) * g.iktx + k]);
+  }
+  __syncthreads();
+  fft_lds<N, true, S, NT>(lds, tw);
+  for (int e = threadIdx.x; e < S * N; e += NT) {
+    const int s = e / N, x = e % N;
+    const double2 v = lds[x * S + s];
+    r[(row0 + 2 * s) * ld + x] = v.x;
+    r[(row0 + 2 * s + 1) * ld + x] = v.y;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+template <int N>
+struct L {
+  using C = Cfg<N>;
+  static hipError_t col_inv(const Dev& d, hipStream_t st, const double2* F, double2* out, int W, int box, int mul,
+                            int mask, int nz) {
+    dim3 grid((W + C::TXI - 1) / C::TXI, nz);
+    hipLaunchKernelGGL((k_col_inv<N, C::TXI>), grid, dim3(NT), 0, st, d.g, d.tw, F, out, W, box, mul, mask);
+    return hipGetLastError();
+  }
+  static hipError_t row_nl(const Dev& d, hipStream_t st, int Win) {
+    dim3 grid(d.g.n2 * d.g.n3 / C::PR);
+    const double scale = 1.0 / ((double)d.g.n1 * (double)d.g.n2);
+    hipLaunchKernelGGL((k_row_nl<N, C::PR>), grid, dim3(NT), 0, st, d, Win, scale);
+    return hipGetLastError();
+  }
+  static hipError_t col_upd(const Dev& d, hipStream_t st, int euler, int has_dq) {
+    dim3 grid((d.g.KX + C::TXU - 1) / C::TXU, d.g.n3, 3);
+    hipLaunchKernelGGL((k_col_upd<N, C::TXU>), grid, dim3(NT), 0, st, d, euler, has_dq);
+    return hipGetLastError();
+  }
+  static hipError_t row_qw(const Dev& d, hipStream_t st) {
+    dim3 grid(d.g.n2 * d.g.n3 / C::PR);
+    const double scale = 1.0 / ((double)d.g.n1 * (double)d.g.n2);
+    hipLaunchKernelGGL((k_row_qw<N, C::PR>), grid, dim3(NT), 0, st, d, scale);
+    return hipGetLastError();
+  }
+  static hipError_t col_qw(const Dev& d, hipStream_t st) {
+    dim3 grid((d.g.KX + C::TXU - 1) / C::TXU, d.g.n3);
+    hipLaunchKernelGGL((k_col_qw<N, C::TXU>), grid, dim3(NT), 0, st, d);
+    return hipGetLastError();
+  }
+  static hipError_t r2c(const Dev& d, hipStream_t st, const double* r, int ld, double2* k, int nz) {
+    const double scale = 1.0 / ((double)d.g.n1 * (double)d.g.n2);
+    hipLaunchKernelGGL((k_row_r2c<N, C::PRC>), dim3(d.g.n2 * nz / C::PRC), dim3(NT), 0, st, d.g, d.tw, r, ld,
+                       d.qs, scale);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_col_fwd<N, C::TXI>), dim3((d.g.iktx + C::TXI - 1) / C::TXI, nz), dim3(NT), 0, st, d.g,
+                       d.tw, d.qs, k);
+    return hipGetLastError();
+  }
+  static hipError_t c2r(const Dev& d, hipStream_t st, const double2* k, double* r, int ld, int nz) {
+    hipError_t e = col_inv(d, st, k, d.qs, d.g.iktx, 0, 0, 0, nz);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_row_c2r<N, C::PRC>), dim3(d.g.n2 * nz / C::PRC), dim3(NT), 0, st, d.g, d.tw, d.qs, r, ld);
+    return hipGetLastError();
+  }
+};
+
+#define QG_DISPATCH(n, call)                 \
+  switch (n) {                               \
+    case 16: return L<16>::call;             \
+    case 32: return L<32>::call;             \
+    case 64: return L<64>::call;             \
+    case 128: return L<128>::call;           \
+    case 256: return L<256>::call;           \
+    case 512: return L<512>::call;           \
+    default: return hipErrorInvalidValue;    \
+  }
+
+bool supported_n(int n) { return n == 16 || n == 32 || n == 64 || n == 128 || n == 256 || n == 512; }
+
+hipError_t launch_col_inv(const Dev& d, hipStream_t st, const double2* F, double2* out, int W, int box, int mul,
+                          int mask, int nz) {
+  QG_DISPATCH(d.g.n2, col_inv(d, st, F, out, W, box, mul, mask, nz))
+}
+hipError_t launch_row_nl(const Dev& d, hipStream_t st, int Win) { QG_DISPATCH(d.g.n1, row_nl(d, st, Win)) }
+hipError_t launch_col_upd(const Dev& d, hipStream_t st, int euler, int has_dq) {
+  QG_DISPATCH(d.g.n2, col_upd(d, st, euler, has_dq))
+}
+hipError_t launch_row_qw(const Dev& d, hipStream_t st) { QG_DISPATCH(d.g.n1, row_qw(d, st)) }
+hipError_t launch_col_qw(const Dev& d, hipStream_t st) { QG_DISPATCH(d.g.n2, col_qw(d, st)) }
+hipError_t launch_r2c(const Dev& d, hipStream_t st, const double* r, int ld, double2* k, int nz) {
+  QG_DISPATCH(d.g.n1, r2c(d, st, r, ld, k, nz))
+}
+hipError_t launch_c2r(const Dev& d, hipStream_t st, const double2* k, double* r, int ld, int nz) {
+  QG_DISPATCH(d.g.n1, c2r(d, st, k, r, ld, nz))
+}
+
+hipError_t launch_solve(const Dev& d, hipStream_t st, int f_lo, int f_hi, const double2* rhs_psi) {
+  if (f_hi < f_lo) return hipSuccess;
+  const int ncols = d.g.KX * d.g.nbox;
+  dim3 grid((2 * ncols + NT - 1) / NT, f_hi - f_lo + 1);
+  hipLaunchKernelGGL(k_solve, grid, dim3(NT), 0, st, d, f_lo, rhs_psi);
+  return hipGetLastError();
+}
+hipError_t launch_dq(const Dev& d, hipStream_t st) {
+  const int n = d.g.KX * d.g.nbox * d.g.n3;
+  hipLaunchKernelGGL(k_dq, dim3((n + NT - 1) / NT), dim3(NT), 0, st, d);
+  return hipGetLastError();
+}
+hipError_t launch_clear_outside(const Dev& d, hipStream_t st, double2* f, int all) {
+  const size_t n = (size_t)d.g.ikty * d.g.iktx * d.g.n3;
+  hipLaunchKernelGGL(k_clear, dim3((unsigned)((n + NT - 1) / NT)), dim3(NT), 0, st, d.g, f, all);
+  return hipGetLastError();
+}
+hipError_t launch_init_q(const Dev& d, hipStream_t st, const double2* psi, double2* q) {
+  const size_t n = (size_t)d.g.ikty * d.g.iktx * d.g.n3;
+  hipLaunchKernelGGL(k_init_q, dim3((unsigned)((n + NT - 1) / NT)), dim3(NT), 0, st, d, psi, q);
+  return hipGetLastError();
+}
+hipError_t launch_cz(const Dev& d, hipStream_t st, const double2* A, double2* C) {
+  const size_t n = (size_t)d.g.ikty * d.g.iktx * d.g.n3;
+  hipLaunchKernelGGL(k_cz, dim3((unsigned)((n + NT - 1) / NT)), dim3(NT), 0, st, d, A, C);
+  return hipGetLastError();
+}
+int diag_blocks(const Geo& g) { return (g.KX * g.nbox * g.n3 + NT - 1) / NT; }
+hipError_t launch_diag(const Dev& d, hipStream_t st, int* nblocks) {
+  const int nb = diag_blocks(d.g);
+  *nblocks = nb;
+  hipLaunchKernelGGL(k_diag, dim3(nb), dim3(NT), 0, st, d);
+  return hipGetLastError();
+}
+
+const char* kernel_name(int id) {
+  static const char* names[K_NKERNELS] = {"col_inv", "row_nl", "col_upd", "row_qw", "col_qw", "solve", "aux"};
+  return (id >= 0 && id < K_NKERNELS) ? names[id] : "?";
+}
+
+}  // namespace qg

@@ -1,0 +1,224 @@
+"""Python (ctypes) harness over the C ABI in include/qgybj.h.
+
+The product is the HIP library ``lib/libqgybj.so``; this module only binds
+it for tests, benchmarks and the verification harness.  Nothing here computes
+a time step: if the library is missing, loading fails loudly.
+
+Arrays: spectral fields are numpy complex128 arrays of shape (n3, n2, n1/2+1)
+(byte-identical to the reference's Fortran (iktx, ikty, n3)); physical fields
+are float64 arrays (nz, n2, n1).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .params import QGParams, make_params  # noqa: F401
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libqgybj.so")
+
+QG_OK = 0
+ERRORS = {1: "QG_ERR_ARG", 2: "QG_ERR_HIP", 3: "QG_ERR_NOMEM", 4: "QG_ERR_NONFINITE", 5: "QG_ERR_STATE"}
+KERNEL_NAMES = ["col_inv", "row_nl", "col_upd", "row_qw", "col_qw", "solve", "aux"]
+
+# exported symbols declared in include/qgybj.h
+SYMBOLS = [
+    "qg_create", "qg_destroy", "qg_last_error", "qg_set_state", "qg_set_state_physical", "qg_get_state",
+    "qg_step", "qg_iter", "qg_diag", "qg_init_q", "qg_fft_r2c", "qg_fft_c2r", "qg_device_field", "qg_stream",
+    "qg_sync", "qg_timing", "qg_kernel_times", "qg_kernel_name", "qg_mask", "qg_box", "qg_version",
+]
+
+
+class QGError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class _Params(C.Structure):
+    _fields_ = [
+        ("n1", C.c_int32), ("n2", C.c_int32), ("n3", C.c_int32), ("flags", C.c_int32),
+        ("dt", C.c_double), ("gamma", C.c_double), ("Ro", C.c_double), ("Bu", C.c_double),
+        ("W2F", C.c_double), ("nuz", C.c_double),
+        ("ifac_q", C.POINTER(C.c_double)), ("ifac_w", C.POINTER(C.c_double)),
+        ("N2_u", C.POINTER(C.c_double)), ("N2_s", C.POINTER(C.c_double)),
+    ]
+
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load libqgybj.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} not built: run qg-ybjp-model_amd/build.py")
+    lib = C.CDLL(path)
+    dp = C.POINTER(C.c_double)
+    vp = C.c_void_p
+    lib.qg_create.argtypes = [C.POINTER(_Params), C.POINTER(vp)]
+    lib.qg_destroy.argtypes = [vp]
+    lib.qg_last_error.argtypes = [vp]
+    lib.qg_last_error.restype = C.c_char_p
+    lib.qg_set_state.argtypes = [vp] + [dp] * 9 + [C.c_int32]
+    lib.qg_set_state_physical.argtypes = [vp, dp, dp, dp, C.c_int32]
+    lib.qg_get_state.argtypes = [vp] + [dp] * 11
+    lib.qg_step.argtypes = [vp, C.c_int32]
+    lib.qg_iter.argtypes = [vp, C.POINTER(C.c_int32)]
+    lib.qg_diag.argtypes = [vp, dp]
+    lib.qg_init_q.argtypes = [vp, dp, dp]
+    lib.qg_fft_r2c.argtypes = [vp, dp, C.c_int32, dp, C.c_int32]
+    lib.qg_fft_c2r.argtypes = [vp, dp, dp, C.c_int32, C.c_int32]
+    lib.qg_device_field.argtypes = [vp, C.c_int32, C.POINTER(vp)]
+    lib.qg_stream.argtypes = [vp, C.POINTER(vp)]
+    lib.qg_sync.argtypes = [vp]
+    lib.qg_timing.argtypes = [vp, C.c_int32]
+    lib.qg_kernel_times.argtypes = [vp, dp, C.POINTER(C.c_int32), C.c_int32]
+    lib.qg_kernel_name.argtypes = [C.c_int32]
+    lib.qg_kernel_name.restype = C.c_char_p
+    lib.qg_mask.argtypes = [C.c_int32, C.c_int32, C.POINTER(C.c_int32)]
+    lib.qg_box.argtypes = [C.c_int32, C.c_int32, C.POINTER(C.c_int32)]
+    lib.qg_version.restype = C.c_char_p
+    _lib = lib
+    return lib
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def mask(n1: int) -> np.ndarray:
+    """The library's dealiasing mask, shape (ikty, iktx) (no GPU needed)."""
+    lib = load()
+    L = np.zeros((n1, n1 // 2 + 1), dtype=np.int32)
+    rc = lib.qg_mask(n1, n1, L.ctypes.data_as(C.POINTER(C.c_int32)))
+    if rc:
+        raise QGError(rc, "qg_mask")
+    return L
+
+
+def box(n1: int):
+    lib = load()
+    out = (C.c_int32 * 4)()
+    rc = lib.qg_box(n1, n1, out)
+    if rc:
+        raise QGError(rc, "qg_box")
+    return tuple(out)
+
+
+class Member:
+    """One QG-YBJ+ member on the current HIP device (a qg_ctx)."""
+
+    FIELDS = ["psik", "qk", "qok", "BRk", "BIk", "BRok", "BIok", "ARk", "AIk"]
+
+    def __init__(self, P: QGParams):
+        self.lib = load()
+        self.P = P
+        self.n1, self.n3 = P.n1, P.n3
+        self.iktx = P.n1 // 2 + 1
+        self.shape = (P.n3, P.n1, self.iktx)
+        self._keep = [np.ascontiguousarray(a, dtype=np.float64) for a in (P.ifac_q, P.ifac_w, P.N2u, P.N2s)]
+        prm = _Params(P.n1, P.n1, P.n3, P.flags, P.dt, P.gamma, P.Ro, P.Bu, P.W2F, P.nuz,
+                      *[_dp(a) for a in self._keep])
+        self._h = C.c_void_p()
+        rc = self.lib.qg_create(C.byref(prm), C.byref(self._h))
+        if rc:
+            raise QGError(rc, "qg_create failed")
+
+    def _chk(self, rc, what):
+        if rc:
+            raise QGError(rc, f"{what}: {self.lib.qg_last_error(self._h).decode()}")
+
+    def close(self):
+        if self._h:
+            self.lib.qg_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _c(self, a):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a, dtype=np.complex128)
+        assert a.shape == self.shape, (a.shape, self.shape)
+        return a
+
+    def set_state(self, psik=None, qk=None, qok=None, BRk=None, BIk=None, BRok=None, BIok=None,
+                  ARk=None, AIk=None, iter=0):
+        arrs = [self._c(a) for a in (psik, qk, qok, BRk, BIk, BRok, BIok, ARk, AIk)]
+        self._chk(self.lib.qg_set_state(self._h, *[_dp(a) for a in arrs], int(iter)), "qg_set_state")
+
+    def set_state_physical(self, psi_r, BR_r, BI_r):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (psi_r, BR_r, BI_r)]
+        for a in arrs:
+            assert a.shape == (self.n3, self.n1, self.n1)
+        self._chk(self.lib.qg_set_state_physical(self._h, *[_dp(a) for a in arrs], self.n1),
+                  "qg_set_state_physical")
+
+    def get_state(self, with_c=False) -> dict:
+        out = {k: np.empty(self.shape, np.complex128) for k in self.FIELDS}
+        cr = np.empty(self.shape, np.complex128) if with_c else None
+        ci = np.empty(self.shape, np.complex128) if with_c else None
+        self._chk(self.lib.qg_get_state(self._h, *[_dp(out[k]) for k in self.FIELDS], _dp(cr), _dp(ci)),
+                  "qg_get_state")
+        if with_c:
+            out["CRk"] = cr
+            out["CIk"] = ci
+        return out
+
+    def step(self, n=1):
+        self._chk(self.lib.qg_step(self._h, int(n)), "qg_step")
+
+    @property
+    def iter(self) -> int:
+        v = C.c_int32()
+        self._chk(self.lib.qg_iter(self._h, C.byref(v)), "qg_iter")
+        return v.value
+
+    def diag(self) -> np.ndarray:
+        out = np.zeros(6)
+        self._chk(self.lib.qg_diag(self._h, _dp(out)), "qg_diag")
+        return out
+
+    def init_q(self, psik):
+        psik = self._c(psik)
+        q = np.empty(self.shape, np.complex128)
+        self._chk(self.lib.qg_init_q(self._h, _dp(psik), _dp(q)), "qg_init_q")
+        return q
+
+    def r2c(self, r):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        nz = r.shape[0]
+        k = np.empty((nz, self.n1, self.iktx), np.complex128)
+        self._chk(self.lib.qg_fft_r2c(self._h, _dp(r), r.shape[-1], _dp(k), nz), "qg_fft_r2c")
+        return k
+
+    def c2r(self, k, ldx=None):
+        k = np.ascontiguousarray(k, dtype=np.complex128)
+        nz = k.shape[0]
+        ldx = ldx or self.n1
+        r = np.empty((nz, self.n1, ldx), np.float64)
+        self._chk(self.lib.qg_fft_c2r(self._h, _dp(k), _dp(r), ldx, nz), "qg_fft_c2r")
+        return r[..., : self.n1]
+
+    def sync(self):
+        self._chk(self.lib.qg_sync(self._h), "qg_sync")
+
+    def timing(self, on=True):
+        self._chk(self.lib.qg_timing(self._h, 1 if on else 0), "qg_timing")
+
+    def kernel_times(self):
+        ms = np.zeros(7)
+        cnt = np.zeros(7, dtype=np.int32)
+        self._chk(self.lib.qg_kernel_times(self._h, _dp(ms), cnt.ctypes.data_as(C.POINTER(C.c_int32)), 7),
+                  "qg_kernel_times")
+        return {KERNEL_NAMES[i]: (float(ms[i]), int(cnt[i])) for i in range(7)}
