@@ -465,6 +465,152 @@ __global__ __launch_bounds__(NT) void k_solve(Dev d, int f_lo, const double2* __
 }
 
 // ---------------------------------------------------------------------------
+// solve_p: the same systems with the z range of every column split into P
+// chunks of M levels (one wave per chunk, 32 columns x {Re, Im} per wave).
+// Each chunk eliminates its own block in registers (DGTSV order, no
+// interchanges) for three right-hand sides: its data g and the two coupling
+// spikes v (from the row above the chunk) and w (from the row below), so that
+// x = g - x_above * v - x_below * w.  The 2P boundary unknowns of a column form
+// a block-tridiagonal system with 2x2 blocks, solved by one lane per column
+// in LDS; each chunk then reconstructs its levels.  Padding rows beyond n3
+// are identity rows decoupled from the system.
+// ---------------------------------------------------------------------------
+template <int M, int P, int LPC>
+__global__ __launch_bounds__(LPC * P) void k_solve_p(Dev d, int f_lo, const double2* __restrict__ rhs_psi) {
+  const Geo& g = d.g;
+  // LPC lanes per chunk: LPC/2 columns x {Re, Im}
+  const int lane = threadIdx.x % LPC, c = threadIdx.x / LPC;
+  const int f = f_lo + blockIdx.y;
+  const int ncols = g.KX * g.nbox;
+  const int col = blockIdx.x * (LPC / 2) + (lane >> 1), part = lane & 1;
+  const bool valid = col < ncols;
+  int kx = 0, iky = 0, kh2 = 0;
+  if (valid) {
+    kx = col % g.KX;
+    iky = box_row(col / g.KX, g);
+    const int ky = ky_of(iky, g.n2);
+    kh2 = kx * kx + ky * ky;
+  }
+  const bool active = valid && kh2 != 0 && lmask(kx, iky, g);
+  const double2* rhs = f == 0 ? rhs_psi : (f == 1 ? d.BR : d.BI);
+  double2* out = f == 0 ? d.psi : (f == 1 ? d.AR : d.AI);
+  const double* __restrict__ rin = reinterpret_cast<const double*>(rhs) + part;
+  double* __restrict__ xo = reinterpret_cast<double*>(out) + part;
+  const size_t plane2 = (size_t)2 * g.ikty * g.iktx;
+  const size_t c2 = (size_t)2 * ((size_t)iky * g.iktx + kx);
+  const int n3 = g.n3;
+  const double dz = d.dz;
+  // inactive columns get a regular (kh2 = 1) system so that no lane divides
+  // by zero; their output is forced to 0
+  const double kh2d = active ? (double)kh2 : 1.0;
+  const double kt = f == 0 ? (kh2d * dz) * dz : ((kh2d * dz) * dz) / 4.0;
+  const double rs = f == 0 ? dz * dz : (dz * dz) * d.Bu;
+  const int s = c * M;
+
+  // local elimination for g (data), v (top spike), w (bottom spike)
+  double dp[M], gv[M], vv[M], wv[M];
+#pragma unroll
+  for (int i = 0; i < M; ++i) {
+    const int z = s + i;
+    gv[i] = (active && z < n3) ? rs * rin[z * plane2 + c2] : 0.0;
+  }
+  const double ltop = (s > 0 && s < n3) ? d.dl[s - 1] : 0.0;          // coupling of row s to row s-1
+  const int e = s + M - 1;
+  const double ubot = (e < n3 - 1) ? d.du[e] : 0.0;                  // coupling of row e to row e+1
+#pragma unroll
+  for (int i = 0; i < M; ++i) {
+    const int z = s + i;
+    const double dz_ = z < n3 ? -(d.dsum[z] + kt) : 1.0;
+    if (i == 0) {
+      dp[0] = dz_;
+      vv[0] = ltop;
+      wv[0] = (M == 1) ? ubot : 0.0;
+    } else {
+      const double lz = z < n3 ? d.dl[z - 1] : 0.0;
+      const double uz = (z - 1 < n3 - 1) ? d.du[z - 1] : 0.0;
+      const double fact = lz / dp[i - 1];
+      dp[i] = dz_ - fact * uz;
+      gv[i] = gv[i] - fact * gv[i - 1];
+      vv[i] = -fact * vv[i - 1];
+      wv[i] = (i == M - 1) ? ubot : 0.0;
+    }
+  }
+  {
+    const double inv = 1.0 / dp[M - 1];
+    gv[M - 1] *= inv;
+    vv[M - 1] *= inv;
+    wv[M - 1] *= inv;
+  }
+#pragma unroll
+  for (int i = M - 2; i >= 0; --i) {
+    const int z = s + i;
+    const double uz = z < n3 - 1 ? d.du[z] : 0.0;
+    const double inv = 1.0 / dp[i];
+    gv[i] = (gv[i] - uz * gv[i + 1]) * inv;
+    vv[i] = (vv[i] - uz * vv[i + 1]) * inv;
+    wv[i] = (wv[i] - uz * wv[i + 1]) * inv;
+  }
+  // reduced system: y_c = (T_c, B_c) = (x_s, x_e);
+  //   T_c + v_s B_{c-1} + w_s T_{c+1} = g_s ;  B_c + v_e B_{c-1} + w_e T_{c+1} = g_e
+  __shared__ double red[P][6][LPC];
+  red[c][0][lane] = gv[0];
+  red[c][1][lane] = gv[M - 1];
+  red[c][2][lane] = vv[0];
+  red[c][3][lane] = vv[M - 1];
+  red[c][4][lane] = wv[0];
+  red[c][5][lane] = wv[M - 1];
+  __syncthreads();
+  if (c == 0) {
+    // Block Thomas with A_b = [[0,vs],[0,ve]], C_b = [[ws,0],[we,0]].  Every
+    // eliminated diagonal block has the form [[D0,0],[D2,1]]; (h0, h1, D0, D2)
+    // of block b overwrite (g_s, g_e, v_s, v_e) of chunk b in LDS.
+    double D0p = 1.0, D2p = 0.0;
+    double h0p = red[0][0][lane], h1p = red[0][1][lane];
+    red[0][2][lane] = 1.0;
+    red[0][3][lane] = 0.0;
+#pragma unroll 1
+    for (int b = 1; b < P; ++b) {
+      const double i10 = -D2p / D0p;  // second row of the inverse: (i10, 1)
+      const double vs = red[b][2][lane], ve = red[b][3][lane];
+      const double ws = red[b - 1][4][lane], we = red[b - 1][5][lane];
+      const double m00 = vs * i10, m01 = vs, m10 = ve * i10, m11 = ve;
+      const double D0 = 1.0 - (m00 * ws + m01 * we);
+      const double D2 = -(m10 * ws + m11 * we);
+      const double h0 = red[b][0][lane] - (m00 * h0p + m01 * h1p);
+      const double h1 = red[b][1][lane] - (m10 * h0p + m11 * h1p);
+      red[b][0][lane] = h0;
+      red[b][1][lane] = h1;
+      red[b][2][lane] = D0;
+      red[b][3][lane] = D2;
+      D0p = D0; D2p = D2; h0p = h0; h1p = h1;
+    }
+    double yT = 0.0;  // T of block b+1
+#pragma unroll 1
+    for (int b = P - 1; b >= 0; --b) {
+      double t0 = red[b][0][lane], t1 = red[b][1][lane];
+      if (b < P - 1) {
+        t0 -= red[b][4][lane] * yT;
+        t1 -= red[b][5][lane] * yT;
+      }
+      const double T = t0 / red[b][2][lane];
+      const double B = t1 - red[b][3][lane] * T;
+      red[b][0][lane] = T;   // solution (T_b, B_b) into slots 0, 1
+      red[b][1][lane] = B;
+      yT = T;
+    }
+  }
+  __syncthreads();
+  if (!valid) return;
+  const double xl = c > 0 ? red[c - 1][1][lane] : 0.0;      // x at row s-1
+  const double xr = c < P - 1 ? red[c + 1][0][lane] : 0.0;  // x at row e+1
+#pragma unroll
+  for (int i = 0; i < M; ++i) {
+    const int z = s + i;
+    if (z < n3) xo[z * plane2 + c2] = active ? gv[i] - xl * vv[i] - xr * wv[i] : 0.0;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // vertical diffusion of the old q (derivatives.f90:1877-1926), box modes
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(NT) void k_dq(Dev d) {
@@ -759,10 +905,27 @@ hipError_t launch_c2r(const Dev& d, hipStream_t st, const double2* k, double* r,
   QG_DISPATCH(d.g.n1, c2r(d, st, k, r, ld, nz))
 }
 
+template <int M, int P, int LPC>
+static hipError_t solve_p(const Dev& d, hipStream_t st, int f_lo, int nf, const double2* rhs_psi) {
+  const int ncols = d.g.KX * d.g.nbox;
+  constexpr int CPB = LPC / 2;
+  dim3 grid((ncols + CPB - 1) / CPB, nf);
+  hipLaunchKernelGGL((k_solve_p<M, P, LPC>), grid, dim3(LPC * P), 0, st, d, f_lo, rhs_psi);
+  return hipGetLastError();
+}
+
 hipError_t launch_solve(const Dev& d, hipStream_t st, int f_lo, int f_hi, const double2* rhs_psi) {
   if (f_hi < f_lo) return hipSuccess;
+  const int nf = f_hi - f_lo + 1;
+  const int n3 = d.g.n3;
+  if (n3 <= 8) return solve_p<8, 1, 64>(d, st, f_lo, nf, rhs_psi);
+  if (n3 <= 16) return solve_p<8, 2, 64>(d, st, f_lo, nf, rhs_psi);
+  if (n3 <= 32) return solve_p<8, 4, 64>(d, st, f_lo, nf, rhs_psi);
+  if (n3 <= 64) return solve_p<8, 8, 64>(d, st, f_lo, nf, rhs_psi);
+  if (n3 <= 128) return solve_p<8, 16, 64>(d, st, f_lo, nf, rhs_psi);
+  if (n3 <= 256) return solve_p<16, 16, 32>(d, st, f_lo, nf, rhs_psi);
   const int ncols = d.g.KX * d.g.nbox;
-  dim3 grid((2 * ncols + NT - 1) / NT, f_hi - f_lo + 1);
+  dim3 grid((2 * ncols + NT - 1) / NT, nf);
   hipLaunchKernelGGL(k_solve, grid, dim3(NT), 0, st, d, f_lo, rhs_psi);
   return hipGetLastError();
 }

@@ -189,3 +189,12 @@ def test_nonfinite_is_reported():
         m.step(1)
     assert e.value.code == 4
     m.close()
+
+
+@pytest.mark.parametrize("n3", [2, 3, 9, 40, 64, 100, 128, 200, 256])
+def test_vertical_partitions(n3):
+    """Every chunking of the partitioned vertical solve (full and padded
+    chunks, 1 to 16 chunks per column) against the sequential oracle."""
+    P = make_params(16, n3, W2F=0.05, dt=2e-3, stratification="exponential")
+    m, S, g, OP, worst = _run_pair(P, 3, k0=2, check_every=True)
+    m.close()
