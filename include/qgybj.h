@@ -124,8 +124,11 @@ int qg_fft_r2c(qg_ctx* ctx, const double* r, int32_t ldx, double* k, int32_t nz)
 int qg_fft_c2r(qg_ctx* ctx, const double* k, double* r, int32_t ldx, int32_t nz);
 
 /* Device-resident access: device address of a state field (complex FP64,
- * layout as above) and the context's HIP stream (hipStream_t). */
+ * (iktx, ikty, n3) with rows padded to qg_device_pitch complex elements so
+ * that rows start on 128-byte lines) and the context's HIP stream
+ * (hipStream_t). */
 int qg_device_field(qg_ctx* ctx, int32_t which, void** dptr);
+int qg_device_pitch(const qg_ctx* ctx, int32_t* pitch);
 int qg_stream(qg_ctx* ctx, void** stream);
 int qg_sync(qg_ctx* ctx);
 

@@ -25,6 +25,7 @@ Geo make_geo(int n1, int n2, int n3) {
   g.n3 = n3;
   g.iktx = n1 / 2 + 1;
   g.ikty = n2;
+  g.sp = (g.iktx + 7) & ~7;
   // init.f90:50,72: L = 0 where (kh*L1/twopi) > ifix(float(n1)/3.+0.5)-0.5,
   // kh = REAL(4) sqrt(kx^2+ky^2).  The test is monotone in kh2, so it is an
   // integer bound kh2 <= kh2max, found here with the same arithmetic.
@@ -53,6 +54,10 @@ Geo make_geo(int n1, int n2, int n3) {
 struct qg_ctx {
   Dev d{};
   hipStream_t stream = nullptr;
+  // side stream for the B -> A inversions, which only the next step's update
+  // reads: evB marks B^{n+1} written, evA marks A^{n+1} written
+  hipStream_t stream2 = nullptr;
+  hipEvent_t evB = nullptr, evA = nullptr;
   int device = 0;
   int iter = 0;
   int p_valid = 0;        // I0, I2, I4 hold the inverse-y transforms of q, BR, BI
@@ -95,7 +100,19 @@ int dalloc(qg_ctx* c, T** p, size_t n) {
   return QG_OK;
 }
 
-size_t field_elems(const Geo& g) { return (size_t)g.n3 * g.ikty * g.iktx; }
+size_t field_elems(const Geo& g) { return (size_t)g.n3 * g.ikty * g.sp; }
+
+// host (iktx, ikty, nz) complex <-> device rows of pitch sp
+hipError_t h2d_field(const Geo& g, double2* dst, const double* src, int nz, hipStream_t st) {
+  return hipMemcpy2DAsync(dst, (size_t)g.sp * sizeof(double2), src, (size_t)g.iktx * sizeof(double2),
+                          (size_t)g.iktx * sizeof(double2), (size_t)g.ikty * nz, hipMemcpyHostToDevice, st);
+}
+hipError_t d2h_field(const Geo& g, double* dst, const double2* src, int nz, hipStream_t st) {
+  hipError_t e = hipMemcpy2DAsync(dst, (size_t)g.iktx * sizeof(double2), src, (size_t)g.sp * sizeof(double2),
+                                  (size_t)g.iktx * sizeof(double2), (size_t)g.ikty * nz, hipMemcpyDeviceToHost, st);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(st);
+}
 
 // exp(-2 pi i m / TWN), from long double, with exact quadrant symmetry
 void make_twiddles(std::vector<double2>& tw) {
@@ -121,15 +138,16 @@ void make_twiddles(std::vector<double2>& tw) {
 struct Timed {
   qg_ctx* c;
   int id;
+  hipStream_t st;
   hipEvent_t a = nullptr;
-  Timed(qg_ctx* cc, int i) : c(cc), id(i) {
-    if (c->timing && hipEventCreate(&a) == hipSuccess) hipEventRecord(a, c->stream);
+  Timed(qg_ctx* cc, int i, hipStream_t s) : c(cc), id(i), st(s) {
+    if (c->timing && hipEventCreate(&a) == hipSuccess) hipEventRecord(a, st);
   }
   void done() {
     if (a) {
       hipEvent_t b = nullptr;
       if (hipEventCreate(&b) == hipSuccess) {
-        hipEventRecord(b, c->stream);
+        hipEventRecord(b, st);
         c->events.push_back({id, a, b});
       } else {
         hipEventDestroy(a);
@@ -139,13 +157,14 @@ struct Timed {
   }
 };
 
-#define KRUN(c, id, call)                                       \
+#define KRUN_ON(c, s, id, call)                                 \
   do {                                                          \
-    Timed _t((c), (id));                                        \
+    Timed _t((c), (id), (s));                                   \
     hipError_t _e = (call);                                     \
     _t.done();                                                  \
     if (_e != hipSuccess) return fail_hip((c), _e, #call);      \
   } while (0)
+#define KRUN(c, id, call) KRUN_ON(c, (c)->stream, id, call)
 
 int step_once(qg_ctx* c) {
   Dev& d = c->d;
@@ -179,18 +198,30 @@ int step_once(qg_ctx* c) {
     KRUN(c, qg::K_COL_INV, qg::launch_col_inv(d, st, d.BI, d.I[4], W, 0, 0, 0, g.n3));
   }
   KRUN(c, qg::K_ROW_NL, qg::launch_row_nl(d, st, W));
+  // the update reads A^n and overwrites B in place: wait for the previous
+  // step's A inversions on the side stream
+  HIPCHK(c, hipStreamWaitEvent(st, c->evA, 0));
   KRUN(c, qg::K_COL_UPD, qg::launch_col_upd(d, st, euler ? 1 : 0, has_dq));
   c->p_valid = 1;
-  const double2* rhs = d.q;
-  if (!(flags & qg::F_FIXED_FLOW) && !(flags & qg::F_NO_FEEDBACK)) {
-    KRUN(c, qg::K_ROW_QW, qg::launch_row_qw(d, st));
-    KRUN(c, qg::K_COL_QW, qg::launch_col_qw(d, st));
-    rhs = d.qs;
+  if (!(flags & qg::F_PASSIVE)) {
+    // A^{n+1} from B^{n+1} (A_solver_ybj_plus) on the side stream; it is
+    // needed by the next update only
+    HIPCHK(c, hipEventRecord(c->evB, st));
+    HIPCHK(c, hipStreamWaitEvent(c->stream2, c->evB, 0));
+    KRUN_ON(c, c->stream2, qg::K_SOLVE, qg::launch_solve(d, c->stream2, 1, 2, nullptr));
+    HIPCHK(c, hipEventRecord(c->evA, c->stream2));
   }
-  const int f_lo = (flags & qg::F_FIXED_FLOW) ? 1 : 0;
-  const int f_hi = (flags & qg::F_PASSIVE) ? 0 : 2;
-  if (f_lo <= f_hi) KRUN(c, qg::K_SOLVE, qg::launch_solve(d, st, f_lo, f_hi, rhs));
+  if (!(flags & qg::F_FIXED_FLOW)) {
+    const double2* rhs = d.q;
+    if (!(flags & qg::F_NO_FEEDBACK)) {
+      KRUN(c, qg::K_ROW_QW, qg::launch_row_qw(d, st));
+      KRUN(c, qg::K_COL_QW, qg::launch_col_qw(d, st));
+      rhs = d.qs;
+    }
+    KRUN(c, qg::K_SOLVE, qg::launch_solve(d, st, 0, 0, rhs));
+  }
   if (c->clear_pending > 0) {
+    HIPCHK(c, hipStreamWaitEvent(st, c->evA, 0));
     // modes outside the dealiasing box are zero in the reference after the
     // masked update / filter / inversions; the hot loop never touches them
     KRUN(c, qg::K_AUX, qg::launch_clear_outside(d, st, d.q, 0));
@@ -250,11 +281,15 @@ int qg_destroy(qg_ctx* c) {
   if (!c) return QG_OK;
   hipSetDevice(c->device);
   if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->stream2) hipStreamSynchronize(c->stream2);
   for (auto& e : c->events) {
     hipEventDestroy(e.a);
     hipEventDestroy(e.b);
   }
   for (void* p : c->allocs) hipFree(p);
+  if (c->evA) hipEventDestroy(c->evA);
+  if (c->evB) hipEventDestroy(c->evB);
+  if (c->stream2) hipStreamDestroy(c->stream2);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return QG_OK;
@@ -274,6 +309,10 @@ int qg_create(const qg_params* p, qg_ctx** out) {
     return QG_ERR_HIP;
   }
   e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->evA, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->evB, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventRecord(c->evA, c->stream2);
   if (e != hipSuccess) {
     qg_destroy(c);
     return QG_ERR_HIP;
@@ -295,7 +334,7 @@ int qg_create(const qg_params* p, qg_ctx** out) {
   for (auto f : fields)
     if ((rc = dalloc(c, f, S)) != QG_OK) { qg_destroy(c); return rc; }
   for (int i = 0; i < 6; ++i)
-    if ((rc = dalloc(c, &d.I[i], (size_t)g.n3 * g.n2 * g.iktx)) != QG_OK) { qg_destroy(c); return rc; }
+    if ((rc = dalloc(c, &d.I[i], (size_t)g.n3 * g.n2 * g.sp)) != QG_OK) { qg_destroy(c); return rc; }
   const size_t ncols = (size_t)g.KX * g.nbox;
   if ((rc = dalloc(c, &d.scratch, 3 * (size_t)g.n3 * ncols * 2)) != QG_OK) { qg_destroy(c); return rc; }
   double2* tw = nullptr;
@@ -373,12 +412,11 @@ int qg_set_state(qg_ctx* c, const double* psik, const double* qk, const double* 
   if (rc) return rc;
   if (iter < 0) return fail(c, QG_ERR_ARG, "iter < 0");
   Dev& d = c->d;
-  const size_t bytes = field_elems(d.g) * sizeof(double2);
   const double* src[] = {psik, qk, qok, BRk, BIk, BRok, BIok, ARk, AIk};
   double2* dst[] = {d.psi, d.q, d.qo, d.BR, d.BI, d.BRo, d.BIo, d.AR, d.AI};
   for (int i = 0; i < 9; ++i) {
-    if (src[i]) HIPCHK(c, hipMemcpyAsync(dst[i], src[i], bytes, hipMemcpyHostToDevice, c->stream));
-    else HIPCHK(c, qg::launch_clear_outside(d, c->stream, dst[i], 1));
+    HIPCHK(c, qg::launch_clear_outside(d, c->stream, dst[i], 1));
+    if (src[i]) HIPCHK(c, h2d_field(d.g, dst[i], src[i], d.g.n3, c->stream));
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->iter = iter;
@@ -424,21 +462,17 @@ int qg_get_state(qg_ctx* c, double* psik, double* qk, double* qok, double* BRk, 
   int rc = check_ctx(c);
   if (rc) return rc;
   Dev& d = c->d;
-  const size_t bytes = field_elems(d.g) * sizeof(double2);
-  HIPCHK(c, hipStreamSynchronize(c->stream));
   double* dst[] = {psik, qk, qok, BRk, BIk, BRok, BIok, ARk, AIk};
   const double2* src[] = {d.psi, d.q, d.qo, d.BR, d.BI, d.BRo, d.BIo, d.AR, d.AI};
   for (int i = 0; i < 9; ++i)
-    if (dst[i]) HIPCHK(c, hipMemcpy(dst[i], src[i], bytes, hipMemcpyDeviceToHost));
+    if (dst[i]) HIPCHK(c, d2h_field(d.g, dst[i], src[i], d.g.n3, c->stream));
   if (CRk) {
     HIPCHK(c, qg::launch_cz(d, c->stream, d.AR, d.qs));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    HIPCHK(c, hipMemcpy(CRk, d.qs, bytes, hipMemcpyDeviceToHost));
+    HIPCHK(c, d2h_field(d.g, CRk, d.qs, d.g.n3, c->stream));
   }
   if (CIk) {
     HIPCHK(c, qg::launch_cz(d, c->stream, d.AI, d.qs));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    HIPCHK(c, hipMemcpy(CIk, d.qs, bytes, hipMemcpyDeviceToHost));
+    HIPCHK(c, d2h_field(d.g, CIk, d.qs, d.g.n3, c->stream));
   }
   return QG_OK;
 }
@@ -453,6 +487,7 @@ int qg_step(qg_ctx* c, int32_t nsteps) {
     if (rc) return rc;
   }
   int flag = 0;
+  HIPCHK(c, hipStreamWaitEvent(c->stream, c->evA, 0));  // join the side stream
   HIPCHK(c, hipMemcpyAsync(&flag, c->d.flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (flag) return fail(c, QG_ERR_NONFINITE, "non-finite value in the updated fields");
@@ -487,11 +522,10 @@ int qg_init_q(qg_ctx* c, const double* psik, double* qk) {
   if (rc) return rc;
   if (!psik || !qk) return fail(c, QG_ERR_ARG, "null array");
   Dev& d = c->d;
-  const size_t bytes = field_elems(d.g) * sizeof(double2);
-  HIPCHK(c, hipMemcpyAsync(d.qs, psik, bytes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, qg::launch_clear_outside(d, c->stream, d.qs, 1));
+  HIPCHK(c, h2d_field(d.g, d.qs, psik, d.g.n3, c->stream));
   HIPCHK(c, qg::launch_init_q(d, c->stream, d.qs, d.dq));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  HIPCHK(c, hipMemcpy(qk, d.dq, bytes, hipMemcpyDeviceToHost));
+  HIPCHK(c, d2h_field(d.g, qk, d.dq, d.g.n3, c->stream));
   return QG_OK;
 }
 
@@ -506,9 +540,7 @@ int qg_fft_r2c(qg_ctx* c, const double* r, int32_t ldx, double* k, int32_t nz) {
   HIPCHK(c, hipMalloc(&tmp, nr * sizeof(double)));
   hipError_t e = hipMemcpyAsync(tmp, r, nr * sizeof(double), hipMemcpyHostToDevice, c->stream);
   if (e == hipSuccess) e = qg::launch_r2c(d, c->stream, tmp, ldx, d.dq, nz);
-  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  if (e == hipSuccess)
-    e = hipMemcpy(k, d.dq, (size_t)nz * g.ikty * g.iktx * sizeof(double2), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = d2h_field(g, k, d.dq, nz, c->stream);
   hipFree(tmp);
   if (e != hipSuccess) return fail_hip(c, e, "qg_fft_r2c");
   c->p_valid = 0;
@@ -525,8 +557,7 @@ int qg_fft_c2r(qg_ctx* c, const double* k, double* r, int32_t ldx, int32_t nz) {
   double* tmp = nullptr;
   HIPCHK(c, hipMalloc(&tmp, nr * sizeof(double)));
   hipError_t e = hipMemsetAsync(tmp, 0, nr * sizeof(double), c->stream);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(d.dq, k, (size_t)nz * g.ikty * g.iktx * sizeof(double2), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = h2d_field(g, d.dq, k, nz, c->stream);
   if (e == hipSuccess) e = qg::launch_c2r(d, c->stream, d.dq, tmp, ldx, nz);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e == hipSuccess) e = hipMemcpy(r, tmp, nr * sizeof(double), hipMemcpyDeviceToHost);
@@ -541,6 +572,12 @@ int qg_device_field(qg_ctx* c, int32_t which, void** dptr) {
   Dev& d = c->d;
   double2* f[] = {d.psi, d.q, d.qo, d.BR, d.BI, d.BRo, d.BIo, d.AR, d.AI};
   *dptr = f[which];
+  return QG_OK;
+}
+
+int qg_device_pitch(const qg_ctx* c, int32_t* pitch) {
+  if (!c || !pitch) return QG_ERR_ARG;
+  *pitch = c->d.g.sp;
   return QG_OK;
 }
 

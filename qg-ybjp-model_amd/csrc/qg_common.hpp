@@ -14,6 +14,8 @@ namespace qg {
 struct Geo {
   int n1, n2, n3;
   int iktx, ikty;
+  int sp;      // device row pitch in complex elements: iktx rounded up to a
+               // multiple of 8 so that every 8-column tile is one 128-byte line
   int kmax;    // largest |k| component with L = 1
   int KX;      // kmax + 1 columns kept
   int nlo;     // rows 0..kmax     (ky = 0..kmax)
@@ -56,9 +58,9 @@ struct Dev {
   Geo g;
   int flags;
   double dt, gamma, Ro, Bu, W2F, nuz, dz;
-  // state (full layout, n3 levels)
+  // state ((iktx, ikty, n3) with row pitch sp)
   double2 *psi, *q, *qo, *BR, *BI, *BRo, *BIo, *AR, *AI;
-  // work: q* (and temporaries), dq, intermediates I0..I5 (pitch iktx, n2 rows/level)
+  // work: q* (and temporaries), dq, intermediates I0..I5 (pitch sp, n2 rows/level)
   double2 *qs, *dq;
   double2* I[6];
   double* scratch;  // Thomas elimination diagonals, 3 * n3 * ncols * 2

@@ -76,12 +76,12 @@ __global__ __launch_bounds__(NT) void k_col_inv(Geo g, const double2* __restrict
                                                  double2* __restrict__ out, int W, int box, int mul, int mask) {
   __shared__ double2 lds[N * TX];
   const int kx0 = blockIdx.x * TX, z = blockIdx.y;
-  const double2* Fz = F + (size_t)z * g.ikty * g.iktx;
+  const double2* Fz = F + (size_t)z * g.ikty * g.sp;
   for (int e = threadIdx.x; e < N * TX; e += NT) {
     const int i = e / TX, c = e % TX, kx = kx0 + c;
     double2 v = zc();
     if (kx < W && (!box || in_box_row(i, g)) && (!mask || lmask(kx, i, g))) {
-      v = Fz[(size_t)i * g.iktx + kx];
+      v = Fz[(size_t)i * g.sp + kx];
       const double ky = (double)ky_of(i, g.n2);
       if (mul == 1) v = mul_iky(v, ky);
       else if (mul == 2) v = cscale(v, -(ky * ky));
@@ -94,10 +94,10 @@ __global__ __launch_bounds__(NT) void k_col_inv(Geo g, const double2* __restrict
     __syncthreads();
   }
   fft_lds<N, true, TX, NT>(lds, tw);
-  double2* oz = out + (size_t)z * g.n2 * g.iktx;
+  double2* oz = out + (size_t)z * g.n2 * g.sp;
   for (int e = threadIdx.x; e < N * TX; e += NT) {
     const int i = e / TX, c = e % TX, kx = kx0 + c;
-    if (kx < W) oz[(size_t)i * g.iktx + kx] = lds[i * TX + c];
+    if (kx < W) oz[(size_t)i * g.sp + kx] = lds[i * TX + c];
   }
 }
 
@@ -121,7 +121,7 @@ __global__ __launch_bounds__(NT) void k_row_nl(Dev d, int Win, double scale) {
   double2* const* I = d.I;
   for (int e = threadIdx.x; e < P * H; e += NT) {
     const int p = e / H, k = e % H;
-    const size_t off = (row0 + p) * g.iktx + k;
+    const size_t off = (row0 + p) * g.sp + k;
     double2 ps = zc(), py = zc(), pyy = zc(), pq = zc(), pbr = zc(), pbi = zc();
     if (k < Win) {
       ps = I[1][off]; py = I[3][off]; pyy = I[5][off];
@@ -166,7 +166,7 @@ __global__ __launch_bounds__(NT) void k_row_nl(Dev d, int Win, double scale) {
   const bool lin = (d.flags & F_LINEAR) != 0, pas = (d.flags & F_PASSIVE) != 0;
   for (int e = threadIdx.x; e < P * g.KX; e += NT) {
     const int p = e / g.KX, k = e % g.KX;
-    const size_t off = (row0 + p) * g.iktx + k;
+    const size_t off = (row0 + p) * g.sp + k;
     double2 Fuq, Fvq, FuR, FvR, FuI, FvI, FzR, FzI;
     split_pair<N>(lds, S2, k, 4 * p + 0, Fuq, Fvq);
     split_pair<N>(lds, S2, k, 4 * p + 1, FuR, FvR);
@@ -203,11 +203,11 @@ __global__ __launch_bounds__(NT) void k_col_upd(Dev d, int euler, int has_dq) {
   const int kx0 = blockIdx.x * TX, z = blockIdx.y, f = blockIdx.z;
   double2* W1 = d.I[2 * f];
   double2* W2 = d.I[2 * f + 1];
-  const size_t pz = (size_t)z * g.n2 * g.iktx;
+  const size_t pz = (size_t)z * g.n2 * g.sp;
   for (int e = threadIdx.x; e < N * S; e += NT) {
     const int i = e / S, s = e % S, c = s % TX, kx = kx0 + c;
     double2 v = zc();
-    if (kx < g.KX) v = (s < TX ? W1 : W2)[pz + (size_t)i * g.iktx + kx];
+    if (kx < g.KX) v = (s < TX ? W1 : W2)[pz + (size_t)i * g.sp + kx];
     lds[i * S + s] = v;
   }
   __syncthreads();
@@ -221,7 +221,7 @@ __global__ __launch_bounds__(NT) void k_col_upd(Dev d, int euler, int has_dq) {
   const double* e2t = f == 0 ? d.e2q : d.e2w;
   const double cd = 0.5 / (d.Bu * d.Ro);
   const double sgn = f == 1 ? 1.0 : -1.0;  // +c kh2 AI for BR, -c kh2 AR for BI
-  const size_t sz = (size_t)z * g.ikty * g.iktx;
+  const size_t sz = (size_t)z * g.ikty * g.sp;
   double2 xn[MAXE];
   bool bad = false;
 #pragma unroll
@@ -234,8 +234,8 @@ __global__ __launch_bounds__(NT) void k_col_upd(Dev d, int euler, int has_dq) {
         const double ky = (double)ky_of(iky, g.n2);
         const double2 F1 = lds[iky * S + c], F2 = lds[iky * S + TX + c];
         const double2 t = make_double2(F1.x - ky * F2.y, F1.y + ky * F2.x);
-        const size_t kidx = (size_t)iky * g.iktx + kx;
-        const size_t idx = sz + kidx;
+        const size_t kidx = (size_t)iky * g.iktx + kx;  // (iky, kx) tables are unpadded
+        const size_t idx = sz + (size_t)iky * g.sp + kx;
         double2 xnew = zc(), xonew = zc();
         if (lmask(kx, iky, g)) {
           const double2 x = X[idx];
@@ -306,8 +306,8 @@ __global__ __launch_bounds__(NT) void k_col_upd(Dev d, int euler, int has_dq) {
   for (int e = threadIdx.x; e < N * TX; e += NT) {
     const int i = e / TX, c = e % TX, kx = kx0 + c;
     if (kx < g.KX) {
-      W1[pz + (size_t)i * g.iktx + kx] = lds[i * S + c];
-      if (f > 0) W2[pz + (size_t)i * g.iktx + kx] = lds[i * S + TX + c];
+      W1[pz + (size_t)i * g.sp + kx] = lds[i * S + c];
+      if (f > 0) W2[pz + (size_t)i * g.sp + kx] = lds[i * S + TX + c];
     }
   }
 }
@@ -327,7 +327,7 @@ __global__ __launch_bounds__(NT) void k_row_qw(Dev d, double scale) {
   double2* const* I = d.I;
   for (int e = threadIdx.x; e < P * H; e += NT) {
     const int p = e / H, k = e % H;
-    const size_t off = (row0 + p) * g.iktx + k;
+    const size_t off = (row0 + p) * g.sp + k;
     double2 br = zc(), bry = zc(), bi = zc(), biy = zc();
     if (k < g.KX) { br = I[2][off]; bry = I[3][off]; bi = I[4][off]; biy = I[5][off]; }
     const double kk = (double)k;
@@ -361,7 +361,7 @@ __global__ __launch_bounds__(NT) void k_row_qw(Dev d, double scale) {
   fft_lds<N, false, S2, NT>(lds, d.tw);
   for (int e = threadIdx.x; e < P * g.KX; e += NT) {
     const int p = e / g.KX, k = e % g.KX;
-    const size_t off = (row0 + p) * g.iktx + k;
+    const size_t off = (row0 + p) * g.sp + k;
     double2 FJ, FM;
     split_pair<N>(lds, S2, k, p, FJ, FM);
     const double c4 = 0.25 * (double)k * (double)k;
@@ -381,20 +381,20 @@ __global__ __launch_bounds__(NT) void k_col_qw(Dev d) {
   __shared__ double2 lds[N * S];
   const Geo& g = d.g;
   const int kx0 = blockIdx.x * TX, z = blockIdx.y;
-  const size_t pz = (size_t)z * g.n2 * g.iktx;
+  const size_t pz = (size_t)z * g.n2 * g.sp;
   for (int e = threadIdx.x; e < N * S; e += NT) {
     const int i = e / S, s = e % S, c = s % TX, kx = kx0 + c;
     double2 v = zc();
-    if (kx < g.KX) v = (s < TX ? d.I[3] : d.I[5])[pz + (size_t)i * g.iktx + kx];
+    if (kx < g.KX) v = (s < TX ? d.I[3] : d.I[5])[pz + (size_t)i * g.sp + kx];
     lds[i * S + s] = v;
   }
   __syncthreads();
   fft_lds<N, false, S, NT>(lds, d.tw);
-  const size_t sz = (size_t)z * g.ikty * g.iktx;
+  const size_t sz = (size_t)z * g.ikty * g.sp;
   for (int e = threadIdx.x; e < g.nbox * TX; e += NT) {
     const int ii = e / TX, c = e % TX, kx = kx0 + c, iky = box_row(ii, g);
     if (kx < g.KX) {
-      const size_t idx = sz + (size_t)iky * g.iktx + kx;
+      const size_t idx = sz + (size_t)iky * g.sp + kx;
       double2 out = zc();
       if (lmask(kx, iky, g)) {
         const double ky = (double)ky_of(iky, g.n2);
@@ -431,8 +431,8 @@ __global__ __launch_bounds__(NT) void k_solve(Dev d, int f_lo, const double2* __
   double2* out = f == 0 ? d.psi : (f == 1 ? d.AR : d.AI);
   const double* rin = reinterpret_cast<const double*>(rhs) + part;
   double* xo = reinterpret_cast<double*>(out) + part;
-  const size_t plane2 = (size_t)2 * g.ikty * g.iktx;
-  const size_t c2 = (size_t)2 * ((size_t)iky * g.iktx + kx);
+  const size_t plane2 = (size_t)2 * g.ikty * g.sp;
+  const size_t c2 = (size_t)2 * ((size_t)iky * g.sp + kx);
   const int n3 = g.n3;
   if (kh2 == 0 || !lmask(kx, iky, g)) {
     for (int z = 0; z < n3; ++z) xo[z * plane2 + c2] = 0.0;
@@ -496,8 +496,8 @@ __global__ __launch_bounds__(LPC * P) void k_solve_p(Dev d, int f_lo, const doub
   double2* out = f == 0 ? d.psi : (f == 1 ? d.AR : d.AI);
   const double* __restrict__ rin = reinterpret_cast<const double*>(rhs) + part;
   double* __restrict__ xo = reinterpret_cast<double*>(out) + part;
-  const size_t plane2 = (size_t)2 * g.ikty * g.iktx;
-  const size_t c2 = (size_t)2 * ((size_t)iky * g.iktx + kx);
+  const size_t plane2 = (size_t)2 * g.ikty * g.sp;
+  const size_t c2 = (size_t)2 * ((size_t)iky * g.sp + kx);
   const int n3 = g.n3;
   const double dz = d.dz;
   // inactive columns get a regular (kh2 = 1) system so that no lane divides
@@ -619,8 +619,8 @@ __global__ __launch_bounds__(NT) void k_dq(Dev d) {
   const int e = blockIdx.x * NT + threadIdx.x;
   if (e >= per * g.n3) return;
   const int z = e / per, r = e % per, kx = r % g.KX, iky = box_row(r / g.KX, g);
-  const size_t plane = (size_t)g.ikty * g.iktx;
-  const size_t idx = z * plane + (size_t)iky * g.iktx + kx;
+  const size_t plane = (size_t)g.ikty * g.sp;
+  const size_t idx = z * plane + (size_t)iky * g.sp + kx;
   double2 v = zc();
   if (lmask(kx, iky, g)) {
     const double dz2 = d.dz * d.dz;
@@ -641,23 +641,23 @@ __global__ __launch_bounds__(NT) void k_dq(Dev d) {
 
 // zero every mode outside the box (all = 1: every mode)
 __global__ __launch_bounds__(NT) void k_clear(Geo g, double2* f, int all) {
-  const size_t plane = (size_t)g.ikty * g.iktx;
+  const size_t plane = (size_t)g.ikty * g.sp;
   const size_t e = (size_t)blockIdx.x * NT + threadIdx.x;
   if (e >= plane * g.n3) return;
   const size_t r = e % plane;
-  const int iky = (int)(r / g.iktx), kx = (int)(r % g.iktx);
+  const int iky = (int)(r / g.sp), kx = (int)(r % g.sp);
   if (all || kx >= g.KX || !in_box_row(iky, g)) f[e] = zc();
 }
 
 // init_q (init.f90:591-652): q = T(kh2) psi / dz^2 at L = 1 modes
 __global__ __launch_bounds__(NT) void k_init_q(Dev d, const double2* __restrict__ psi, double2* __restrict__ q) {
   const Geo& g = d.g;
-  const size_t plane = (size_t)g.ikty * g.iktx;
+  const size_t plane = (size_t)g.ikty * g.sp;
   const size_t e = (size_t)blockIdx.x * NT + threadIdx.x;
   if (e >= plane * g.n3) return;
   const int z = (int)(e / plane);
   const size_t r = e % plane;
-  const int iky = (int)(r / g.iktx), kx = (int)(r % g.iktx);
+  const int iky = (int)(r / g.sp), kx = (int)(r % g.sp);
   double2 v = zc();
   if (lmask(kx, iky, g)) {
     const int ky = ky_of(iky, g.n2);
@@ -685,7 +685,7 @@ __global__ __launch_bounds__(NT) void k_init_q(Dev d, const double2* __restrict_
 // C = A_z: forward difference, zero at the top (elliptic.f90:437-450)
 __global__ __launch_bounds__(NT) void k_cz(Dev d, const double2* __restrict__ A, double2* __restrict__ C) {
   const Geo& g = d.g;
-  const size_t plane = (size_t)g.ikty * g.iktx;
+  const size_t plane = (size_t)g.ikty * g.sp;
   const size_t e = (size_t)blockIdx.x * NT + threadIdx.x;
   if (e >= plane * g.n3) return;
   const int z = (int)(e / plane);
@@ -708,8 +708,8 @@ __global__ __launch_bounds__(NT) void k_diag(Dev d) {
   if (e < per * g.n3) {
     const int z = e / per, r = e % per, kx = r % g.KX, iky = box_row(r / g.KX, g);
     if (lmask(kx, iky, g)) {
-      const size_t plane = (size_t)g.ikty * g.iktx;
-      const size_t idx = z * plane + (size_t)iky * g.iktx + kx;
+      const size_t plane = (size_t)g.ikty * g.sp;
+      const size_t idx = z * plane + (size_t)iky * g.sp + kx;
       const int ky = ky_of(iky, g.n2);
       const double kh2 = (double)(kx * kx + ky * ky);
       const double w = (kx == 0 && iky == 0) ? 0.5 : 1.0;
@@ -777,8 +777,8 @@ __global__ __launch_bounds__(NT) void k_row_r2c(Geo g, const double2* __restrict
     const int s = e / H, k = e % H;
     double2 A, B;
     split_pair<N>(lds, S, k, s, A, B);
-    out[(row0 + 2 * s) * g.iktx + k] = cscale(A, scale);
-    out[(row0 + 2 * s + 1) * g.iktx + k] = cscale(B, scale);
+    out[(row0 + 2 * s) * g.sp + k] = cscale(A, scale);
+    out[(row0 + 2 * s + 1) * g.sp + k] = cscale(B, scale);
   }
 }
 
@@ -788,17 +788,17 @@ __global__ __launch_bounds__(NT) void k_col_fwd(Geo g, const double2* __restrict
                                                  double2* __restrict__ F) {
   __shared__ double2 lds[N * TX];
   const int kx0 = blockIdx.x * TX, z = blockIdx.y;
-  const double2* iz = in + (size_t)z * g.n2 * g.iktx;
+  const double2* iz = in + (size_t)z * g.n2 * g.sp;
   for (int e = threadIdx.x; e < N * TX; e += NT) {
     const int i = e / TX, c = e % TX, kx = kx0 + c;
-    lds[i * TX + c] = kx < g.iktx ? iz[(size_t)i * g.iktx + kx] : zc();
+    lds[i * TX + c] = kx < g.iktx ? iz[(size_t)i * g.sp + kx] : zc();
   }
   __syncthreads();
   fft_lds<N, false, TX, NT>(lds, tw);
-  double2* Fz = F + (size_t)z * g.ikty * g.iktx;
+  double2* Fz = F + (size_t)z * g.ikty * g.sp;
   for (int e = threadIdx.x; e < N * TX; e += NT) {
     const int i = e / TX, c = e % TX, kx = kx0 + c;
-    if (kx < g.iktx) Fz[(size_t)i * g.iktx + kx] = lds[i * TX + c];
+    if (kx < g.iktx) Fz[(size_t)i * g.sp + kx] = lds[i * TX + c];
   }
 }
 
@@ -811,7 +811,7 @@ __global__ __launch_bounds__(NT) void k_row_c2r(Geo g, const double2* __restrict
   const size_t row0 = (size_t)blockIdx.x * P;
   for (int e = threadIdx.x; e < S * H; e += NT) {
     const int s = e / H, k = e % H;
-    put_pair<N>(lds, S, k, s, in[(row0 + 2 * s) * g.iktx + k], in[(row0 + 2 * s + 1) * g.iktx + k]);
+    put_pair<N>(lds, S, k, s, in[(row0 + 2 * s) * g.sp + k], in[(row0 + 2 * s + 1) * g.sp + k]);
   }
   __syncthreads();
   fft_lds<N, true, S, NT>(lds, tw);
@@ -935,17 +935,17 @@ hipError_t launch_dq(const Dev& d, hipStream_t st) {
   return hipGetLastError();
 }
 hipError_t launch_clear_outside(const Dev& d, hipStream_t st, double2* f, int all) {
-  const size_t n = (size_t)d.g.ikty * d.g.iktx * d.g.n3;
+  const size_t n = (size_t)d.g.ikty * d.g.sp * d.g.n3;
   hipLaunchKernelGGL(k_clear, dim3((unsigned)((n + NT - 1) / NT)), dim3(NT), 0, st, d.g, f, all);
   return hipGetLastError();
 }
 hipError_t launch_init_q(const Dev& d, hipStream_t st, const double2* psi, double2* q) {
-  const size_t n = (size_t)d.g.ikty * d.g.iktx * d.g.n3;
+  const size_t n = (size_t)d.g.ikty * d.g.sp * d.g.n3;
   hipLaunchKernelGGL(k_init_q, dim3((unsigned)((n + NT - 1) / NT)), dim3(NT), 0, st, d, psi, q);
   return hipGetLastError();
 }
 hipError_t launch_cz(const Dev& d, hipStream_t st, const double2* A, double2* C) {
-  const size_t n = (size_t)d.g.ikty * d.g.iktx * d.g.n3;
+  const size_t n = (size_t)d.g.ikty * d.g.sp * d.g.n3;
   hipLaunchKernelGGL(k_cz, dim3((unsigned)((n + NT - 1) / NT)), dim3(NT), 0, st, d, A, C);
   return hipGetLastError();
 }

@@ -27,7 +27,7 @@ KERNEL_NAMES = ["col_inv", "row_nl", "col_upd", "row_qw", "col_qw", "solve", "au
 # exported symbols declared in include/qgybj.h
 SYMBOLS = [
     "qg_create", "qg_destroy", "qg_last_error", "qg_set_state", "qg_set_state_physical", "qg_get_state",
-    "qg_step", "qg_iter", "qg_diag", "qg_init_q", "qg_fft_r2c", "qg_fft_c2r", "qg_device_field", "qg_stream",
+    "qg_step", "qg_iter", "qg_diag", "qg_init_q", "qg_fft_r2c", "qg_fft_c2r", "qg_device_field", "qg_device_pitch", "qg_stream",
     "qg_sync", "qg_timing", "qg_kernel_times", "qg_kernel_name", "qg_mask", "qg_box", "qg_version",
 ]
 
@@ -75,6 +75,7 @@ def load(path: str = LIB_PATH):
     lib.qg_fft_r2c.argtypes = [vp, dp, C.c_int32, dp, C.c_int32]
     lib.qg_fft_c2r.argtypes = [vp, dp, dp, C.c_int32, C.c_int32]
     lib.qg_device_field.argtypes = [vp, C.c_int32, C.POINTER(vp)]
+    lib.qg_device_pitch.argtypes = [vp, C.POINTER(C.c_int32)]
     lib.qg_stream.argtypes = [vp, C.POINTER(vp)]
     lib.qg_sync.argtypes = [vp]
     lib.qg_timing.argtypes = [vp, C.c_int32]
