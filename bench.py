@@ -50,7 +50,8 @@ def kernel_bytes(n1: int, n3: int):
         "col_upd": 11 * G + 14 * C,  # 6 in + 5 out; q,qo,BR,BRo,BI,BIo,AR,AI in, 6 out
         "row_qw": 6 * G,           # 4 in, 2 out
         "col_qw": 2 * G + 2 * C,   # 2 in, q in, q* out
-        "solve": 6 * C,            # q*, BR, BI in; psi, AR, AI out
+        "solve": 2 * C,            # q* in, psi out
+        "solve_a": 4 * C,          # BR, BI in; AR, AI out (side stream)
     }, {"G": G, "C": C, "KX": KX, "nbox": nbox}
 
 

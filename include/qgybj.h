@@ -134,11 +134,19 @@ int qg_sync(qg_ctx* ctx);
 
 /* Per-kernel timing with HIP events on the context's stream: enable, then
  * read accumulated milliseconds and launch counts per kernel id
- * (0 col_inv, 1 row_nl, 2 col_upd, 3 row_qw, 4 col_qw, 5 solve, 6 aux).
+ * (0 col_inv, 1 row_nl, 2 col_upd, 3 row_qw, 4 col_qw, 5 solve (psi), 6 solve_a
+ * (A, on the side stream), 7 aux).
  * Reading resets the accumulators. */
 int qg_timing(qg_ctx* ctx, int32_t enable);
 int qg_kernel_times(qg_ctx* ctx, double* ms, int32_t* counts, int32_t n);
 const char* qg_kernel_name(int32_t id);
+
+/* Kernel variants of the five transform kernels (ps_inv, row_nl, col_upd,
+ * row_qw, col_qw).  Column kernels: 0 = 8 elements per thread, 1 = 16.  Row
+ * kernels: 0 = 8 per thread in 256-thread blocks, 1 = 16/256, 2 = 8/64,
+ * 3 = 16/64.  Set by the tuner; results agree to FP64 rounding. */
+int qg_set_variants(qg_ctx* ctx, const int32_t* e16, int32_t n);
+int qg_get_variants(const qg_ctx* ctx, int32_t* e16, int32_t n);
 
 /* Host-only helpers (no GPU needed): the dealiasing mask L(iktx, ikty) as the
  * library evaluates it, and the box geometry {kmax, KX, nlo, nhi}. */

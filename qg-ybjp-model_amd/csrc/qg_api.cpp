@@ -188,14 +188,12 @@ int step_once(qg_ctx* c) {
   const int box = full ? 0 : 1;
   // u, v, zeta ingredients from psi (compute_velo derivatives.f90:1302-1311,
   // refraction_waqg :831-845)
-  KRUN(c, qg::K_COL_INV, qg::launch_col_inv(d, st, d.psi, d.I[1], W, box, 0, 1, g.n3));
-  KRUN(c, qg::K_COL_INV, qg::launch_col_inv(d, st, d.psi, d.I[3], W, box, 1, 1, g.n3));
-  KRUN(c, qg::K_COL_INV, qg::launch_col_inv(d, st, d.psi, d.I[5], W, box, 2, 1, g.n3));
+  KRUN(c, qg::K_COL_INV, qg::launch_col_inv(d, st, d.psi, d.I[1], d.I[3], d.I[5], W, box, 1, g.n3));
   if (full) {
     // unmasked fields as the state holds them (the initial B is un-dealiased)
-    KRUN(c, qg::K_COL_INV, qg::launch_col_inv(d, st, d.q, d.I[0], W, 0, 0, 0, g.n3));
-    KRUN(c, qg::K_COL_INV, qg::launch_col_inv(d, st, d.BR, d.I[2], W, 0, 0, 0, g.n3));
-    KRUN(c, qg::K_COL_INV, qg::launch_col_inv(d, st, d.BI, d.I[4], W, 0, 0, 0, g.n3));
+    KRUN(c, qg::K_COL_INV, qg::launch_col_inv(d, st, d.q, d.I[0], nullptr, nullptr, W, 0, 0, g.n3));
+    KRUN(c, qg::K_COL_INV, qg::launch_col_inv(d, st, d.BR, d.I[2], nullptr, nullptr, W, 0, 0, g.n3));
+    KRUN(c, qg::K_COL_INV, qg::launch_col_inv(d, st, d.BI, d.I[4], nullptr, nullptr, W, 0, 0, g.n3));
   }
   KRUN(c, qg::K_ROW_NL, qg::launch_row_nl(d, st, W));
   // the update reads A^n and overwrites B in place: wait for the previous
@@ -208,7 +206,7 @@ int step_once(qg_ctx* c) {
     // needed by the next update only
     HIPCHK(c, hipEventRecord(c->evB, st));
     HIPCHK(c, hipStreamWaitEvent(c->stream2, c->evB, 0));
-    KRUN_ON(c, c->stream2, qg::K_SOLVE, qg::launch_solve(d, c->stream2, 1, 2, nullptr));
+    KRUN_ON(c, c->stream2, qg::K_SOLVE_A, qg::launch_solve(d, c->stream2, 1, 2, nullptr));
     HIPCHK(c, hipEventRecord(c->evA, c->stream2));
   }
   if (!(flags & qg::F_FIXED_FLOW)) {
@@ -572,6 +570,18 @@ int qg_device_field(qg_ctx* c, int32_t which, void** dptr) {
   Dev& d = c->d;
   double2* f[] = {d.psi, d.q, d.qo, d.BR, d.BI, d.BRo, d.BIo, d.AR, d.AI};
   *dptr = f[which];
+  return QG_OK;
+}
+
+int qg_set_variants(qg_ctx* c, const int32_t* e16, int32_t n) {
+  if (!c || !e16 || n < 0) return QG_ERR_ARG;
+  for (int i = 0; i < n && i < 5; ++i) c->d.tune.e16[i] = (e16[i] >= 0 && e16[i] <= 3) ? e16[i] : 0;
+  return QG_OK;
+}
+
+int qg_get_variants(const qg_ctx* c, int32_t* e16, int32_t n) {
+  if (!c || !e16 || n < 0) return QG_ERR_ARG;
+  for (int i = 0; i < n && i < 5; ++i) e16[i] = c->d.tune.e16[i];
   return QG_OK;
 }
 

@@ -53,9 +53,17 @@ enum : int {
   F_FIXED_FLOW = 1, F_NO_FEEDBACK = 2, F_PASSIVE = 4, F_LINEAR = 8, F_NO_DISPERSION = 16, F_INVISCID = 32
 };
 
+// Kernel variant choice per hot kernel (ps_inv, row_nl, col_upd, row_qw,
+// col_qw).  Column kernels: 1 = 16 elements per thread, 0 = 8.  Row kernels:
+// 0 = 8 elements / 256 threads, 1 = 16 / 256, 2 = 8 / 64, 3 = 16 / 64.
+struct Tune {
+  int e16[5];
+};
+
 // Device-side view of one member: state, work buffers and tables.
 struct Dev {
   Geo g;
+  Tune tune;
   int flags;
   double dt, gamma, Ro, Bu, W2F, nuz, dz;
   // state ((iktx, ikty, n3) with row pitch sp)
@@ -75,15 +83,16 @@ struct Dev {
 
 // kernel ids for per-kernel timing
 enum KernelId : int {
-  K_COL_INV = 0, K_ROW_NL, K_COL_UPD, K_ROW_QW, K_COL_QW, K_SOLVE, K_AUX, K_NKERNELS
+  K_COL_INV = 0, K_ROW_NL, K_COL_UPD, K_ROW_QW, K_COL_QW, K_SOLVE, K_SOLVE_A, K_AUX, K_NKERNELS
 };
 const char* kernel_name(int id);
 
 // ---- launchers (qg_kernels.hip); all enqueue on `st` and return hipError_t
-// column inverse transform of one spectral field into an intermediate
-// mul: 0 -> 1, 1 -> i*ky, 2 -> -ky^2; W columns; box rows only if box
-hipError_t launch_col_inv(const Dev& d, hipStream_t st, const double2* F, double2* out, int W, int box, int mul,
-                          int mask, int nz);
+// inverse y transform of one spectral field into up to three intermediates
+// with the multipliers 1 (o0), i*ky (o1), -ky^2 (o2); o1/o2 may be null;
+// W columns; box rows only if box; mask applies L
+hipError_t launch_col_inv(const Dev& d, hipStream_t st, const double2* F, double2* o0, double2* o1, double2* o2,
+                          int W, int box, int mask, int nz);
 hipError_t launch_row_nl(const Dev& d, hipStream_t st, int Win);
 hipError_t launch_col_upd(const Dev& d, hipStream_t st, int euler, int has_dq);
 hipError_t launch_row_qw(const Dev& d, hipStream_t st);

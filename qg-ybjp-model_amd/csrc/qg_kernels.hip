@@ -1,25 +1,9 @@
-// HIP kernels of the QG-YBJ+ time step for gfx950 (FP64 throughout).
-//
-// Data flow of one leapfrog step (spectral fields in the (iktx, ikty, n3)
-// layout, intermediates I0..I5 in the half-transformed (kx, y) layout):
-//
-//   col_inv  : psi^n            -> I1 = Fy^-1[L psi], I3 = Fy^-1[L i ky psi],
-//                                  I5 = Fy^-1[L (-ky^2) psi]
-//   row_nl   : I0..I5           -> u, v, zeta, q, BR, BI rows (c2r in x),
-//                                  the 8 products, r2c in x, pre-combined
-//                                  flux tendencies -> I0..I5
-//   col_upd  : I0..I5 + state   -> r2c in y, leapfrog + integrating factor +
-//                                  Robert filter in place, then the inverse
-//                                  y transforms of the new q, BR, BI (and of
-//                                  i ky BR, i ky BI) -> I0 (q), I2,I3 (BR),
-//                                  I4,I5 (BI)
-//   row_qw   : I2..I5           -> J(BR,BI) and |B|^2 rows -> I3, I5
-//   col_qw   : I3, I5, q^{n+1}  -> q* = q - q^w
-//   solve    : q*, BR, BI       -> psi, AR, AI (Thomas sweep in z per column)
-//
-// The reference computes the same quantities with 24 per-level 2D FFTW
-// transforms, 8 MPI transposes and 6 DGTSV calls per column per step
-// (main_waqg.f90:326-467, derivatives.f90:607-1032, elliptic.f90:145-460).
+// HIP kernels around the QG-YBJ+ step for gfx950 (FP64 throughout): the
+// vertical inversions (psi_solver, A_solver_ybj_plus: elliptic.f90:145-460),
+// vertical diffusion, init_q, C = A_z, the FP64 diagnostics, mode clearing and
+// the raw r2c / c2r transforms of fft.f90:55-107 used for initial conditions
+// and tests.  The transform-method kernels of the step are in
+// qg_step_kernels.hip.
 #include "qg_common.hpp"
 #include "qg_fft.hpp"
 
@@ -70,6 +54,7 @@ constexpr int NT = 256;
 // col_inv: inverse DFT in y of one spectral field, with an optional spectral
 // multiplier, the mask L and the reference's realit fill of (0,-ky)
 // (fft.f90:71-107).  Output: intermediate (kx, y) rows, columns kx < W.
+// Used by the raw c2r entry point; the time step uses k_ps_inv.
 // ---------------------------------------------------------------------------
 template <int N, int TX>
 __global__ __launch_bounds__(NT) void k_col_inv(Geo g, const double2* __restrict__ tw, const double2* __restrict__ F,
@@ -98,314 +83,6 @@ __global__ __launch_bounds__(NT) void k_col_inv(Geo g, const double2* __restrict
   for (int e = threadIdx.x; e < N * TX; e += NT) {
     const int i = e / TX, c = e % TX, kx = kx0 + c;
     if (kx < W) oz[(size_t)i * g.sp + kx] = lds[i * TX + c];
-  }
-}
-
-// ---------------------------------------------------------------------------
-// row_nl: per (z, y) row, c2r in x of u, v, zeta, q, BR, BI, the products of
-// convol_waqg / refraction_waqg (derivatives.f90:661-674, 709-722, 756-769,
-// 862-875), r2c in x and the pre-combination of the flux divergences:
-//   I0 = i kx F(uq)                   I1 = F(vq)
-//   I2 = i kx F(uBR) - F(zeta BI)/2   I3 = F(vBR)
-//   I4 = i kx F(uBI) + F(zeta BR)/2   I5 = F(vBI)
-// so that after the y transform  n_q = I0 + i ky I1,
-// n_BR - r_BI/2 = I2 + i ky I3 and n_BI + r_BR/2 = I4 + i ky I5.
-// ---------------------------------------------------------------------------
-template <int N, int P>
-__global__ __launch_bounds__(NT) void k_row_nl(Dev d, int Win, double scale) {
-  constexpr int S1 = 3 * P, S2 = 4 * P, H = N / 2 + 1;
-  constexpr int PPT = (P * N + NT - 1) / NT;
-  __shared__ double2 lds[N * S2];
-  const Geo& g = d.g;
-  const size_t row0 = (size_t)blockIdx.x * P;
-  double2* const* I = d.I;
-  for (int e = threadIdx.x; e < P * H; e += NT) {
-    const int p = e / H, k = e % H;
-    const size_t off = (row0 + p) * g.sp + k;
-    double2 ps = zc(), py = zc(), pyy = zc(), pq = zc(), pbr = zc(), pbi = zc();
-    if (k < Win) {
-      ps = I[1][off]; py = I[3][off]; pyy = I[5][off];
-      pq = I[0][off]; pbr = I[2][off]; pbi = I[4][off];
-    }
-    const double kk = (double)k, k2 = kk * kk;
-    const double2 U = make_double2(-py.x, -py.y);
-    const double2 V = make_double2(-kk * ps.y, kk * ps.x);
-    const double2 Z = make_double2(-k2 * ps.x + pyy.x, -k2 * ps.y + pyy.y);
-    put_pair<N>(lds, S1, k, 3 * p + 0, U, V);
-    put_pair<N>(lds, S1, k, 3 * p + 1, Z, pq);
-    put_pair<N>(lds, S1, k, 3 * p + 2, pbr, pbi);
-  }
-  __syncthreads();
-  fft_lds<N, true, S1, NT>(lds, d.tw);
-  double2 pr[PPT][4];
-#pragma unroll
-  for (int q = 0; q < PPT; ++q) {
-    const int e = threadIdx.x + q * NT;
-    if (e < P * N) {
-      const int p = e % P, n = e / P;
-      const double2 a = lds[n * S1 + 3 * p], b = lds[n * S1 + 3 * p + 1], c = lds[n * S1 + 3 * p + 2];
-      const double u = a.x, v = a.y, zeta = b.x, qv = b.y, br = c.x, bi = c.y;
-      pr[q][0] = make_double2(u * qv, v * qv);
-      pr[q][1] = make_double2(u * br, v * br);
-      pr[q][2] = make_double2(u * bi, v * bi);
-      pr[q][3] = make_double2(zeta * br, zeta * bi);
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < PPT; ++q) {
-    const int e = threadIdx.x + q * NT;
-    if (e < P * N) {
-      const int p = e % P, n = e / P;
-#pragma unroll
-      for (int c = 0; c < 4; ++c) lds[n * S2 + 4 * p + c] = pr[q][c];
-    }
-  }
-  __syncthreads();
-  fft_lds<N, false, S2, NT>(lds, d.tw);
-  const bool lin = (d.flags & F_LINEAR) != 0, pas = (d.flags & F_PASSIVE) != 0;
-  for (int e = threadIdx.x; e < P * g.KX; e += NT) {
-    const int p = e / g.KX, k = e % g.KX;
-    const size_t off = (row0 + p) * g.sp + k;
-    double2 Fuq, Fvq, FuR, FvR, FuI, FvI, FzR, FzI;
-    split_pair<N>(lds, S2, k, 4 * p + 0, Fuq, Fvq);
-    split_pair<N>(lds, S2, k, 4 * p + 1, FuR, FvR);
-    split_pair<N>(lds, S2, k, 4 * p + 2, FuI, FvI);
-    split_pair<N>(lds, S2, k, 4 * p + 3, FzR, FzI);
-    const double kk = (double)k;
-    double2 w0 = mul_iky(Fuq, kk), w1 = Fvq;
-    double2 w2 = mul_iky(FuR, kk), w3 = FvR;
-    double2 w4v = mul_iky(FuI, kk), w5 = FvI;
-    if (lin) { w0 = zc(); w1 = zc(); w2 = zc(); w3 = zc(); w4v = zc(); w5 = zc(); }
-    if (!pas) {
-      w2 = make_double2(w2.x - 0.5 * FzI.x, w2.y - 0.5 * FzI.y);
-      w4v = make_double2(w4v.x + 0.5 * FzR.x, w4v.y + 0.5 * FzR.y);
-    }
-    I[0][off] = cscale(w0, scale); I[1][off] = cscale(w1, scale);
-    I[2][off] = cscale(w2, scale); I[3][off] = cscale(w3, scale);
-    I[4][off] = cscale(w4v, scale); I[5][off] = cscale(w5, scale);
-  }
-}
-
-// ---------------------------------------------------------------------------
-// col_upd: r2c in y of the pre-combined tendencies of one field, the time
-// update (Euler: main_waqg.f90:251-253; leapfrog + Robert-Asselin filter:
-// :375-377, :400-402, :413-415) in place, then the inverse y transforms of
-// the new field for the next step's products and for the feedback term
-// (derivatives.f90:938-956).  blockIdx.z: 0 = q, 1 = BR, 2 = BI.
-// ---------------------------------------------------------------------------
-template <int N, int TX>
-__global__ __launch_bounds__(NT) void k_col_upd(Dev d, int euler, int has_dq) {
-  constexpr int S = 2 * TX;
-  constexpr int MAXE = (N * TX + NT - 1) / NT;
-  __shared__ double2 lds[N * S];
-  const Geo& g = d.g;
-  const int kx0 = blockIdx.x * TX, z = blockIdx.y, f = blockIdx.z;
-  double2* W1 = d.I[2 * f];
-  double2* W2 = d.I[2 * f + 1];
-  const size_t pz = (size_t)z * g.n2 * g.sp;
-  for (int e = threadIdx.x; e < N * S; e += NT) {
-    const int i = e / S, s = e % S, c = s % TX, kx = kx0 + c;
-    double2 v = zc();
-    if (kx < g.KX) v = (s < TX ? W1 : W2)[pz + (size_t)i * g.sp + kx];
-    lds[i * S + s] = v;
-  }
-  __syncthreads();
-  fft_lds<N, false, S, NT>(lds, d.tw);
-
-  double2* X = f == 0 ? d.q : (f == 1 ? d.BR : d.BI);
-  double2* Xo = f == 0 ? d.qo : (f == 1 ? d.BRo : d.BIo);
-  const double2* Av = f == 1 ? d.AI : d.AR;
-  const bool useA = f > 0 && !(d.flags & (F_NO_DISPERSION | F_PASSIVE));
-  const double* e1t = f == 0 ? d.e1q : d.e1w;
-  const double* e2t = f == 0 ? d.e2q : d.e2w;
-  const double cd = 0.5 / (d.Bu * d.Ro);
-  const double sgn = f == 1 ? 1.0 : -1.0;  // +c kh2 AI for BR, -c kh2 AR for BI
-  const size_t sz = (size_t)z * g.ikty * g.sp;
-  double2 xn[MAXE];
-  bool bad = false;
-#pragma unroll
-  for (int q = 0; q < MAXE; ++q) {
-    const int e = threadIdx.x + q * NT;
-    xn[q] = zc();
-    if (e < g.nbox * TX) {
-      const int ii = e / TX, c = e % TX, kx = kx0 + c, iky = box_row(ii, g);
-      if (kx < g.KX) {
-        const double ky = (double)ky_of(iky, g.n2);
-        const double2 F1 = lds[iky * S + c], F2 = lds[iky * S + TX + c];
-        const double2 t = make_double2(F1.x - ky * F2.y, F1.y + ky * F2.x);
-        const size_t kidx = (size_t)iky * g.iktx + kx;  // (iky, kx) tables are unpadded
-        const size_t idx = sz + (size_t)iky * g.sp + kx;
-        double2 xnew = zc(), xonew = zc();
-        if (lmask(kx, iky, g)) {
-          const double2 x = X[idx];
-          const double e1 = e1t[kidx];
-          const double kh2 = (double)kx * (double)kx + ky * ky;
-          double2 dqv = zc();
-          if (f == 0 && has_dq) dqv = d.dq[idx];
-          double2 av = zc();
-          if (useA) av = Av[idx];
-          if (euler) {
-            if (f == 0) {
-              xnew = make_double2(((x.x - d.dt * t.x) + d.dt * dqv.x) * e1, ((x.y - d.dt * t.y) + d.dt * dqv.y) * e1);
-            } else {
-              const double ck = (d.dt * cd) * kh2;
-              xnew = make_double2(((x.x - d.dt * t.x) - sgn * (ck * av.x)) * e1,
-                                  ((x.y - d.dt * t.y) - sgn * (ck * av.y)) * e1);
-            }
-          } else {
-            const double2 xo = Xo[idx];
-            const double e2 = e2t[kidx];
-            const double tdt = 2.0 * d.dt;
-            if (f == 0) {
-              xnew = make_double2((xo.x * e2 - (tdt * t.x) * e1) + (tdt * dqv.x) * e2,
-                                  (xo.y * e2 - (tdt * t.y) * e1) + (tdt * dqv.y) * e2);
-            } else {
-              const double ck = cd * kh2;
-              xnew = make_double2(xo.x * e2 - (tdt * (t.x + sgn * (ck * av.x))) * e1,
-                                  xo.y * e2 - (tdt * (t.y + sgn * (ck * av.y))) * e1);
-            }
-            xonew = make_double2(x.x + d.gamma * ((xo.x - 2.0 * x.x) + xnew.x),
-                                 x.y + d.gamma * ((xo.y - 2.0 * x.y) + xnew.y));
-          }
-        }
-        X[idx] = xnew;
-        if (!euler) Xo[idx] = xonew;
-        bad |= !(isfinite(xnew.x) && isfinite(xnew.y));
-        xn[q] = xnew;
-      }
-    }
-  }
-  if (bad) atomicOr(d.flag, 1);
-  __syncthreads();
-  // stage the new field (and i ky times it) for the inverse y transforms
-#pragma unroll
-  for (int q = 0; q < MAXE; ++q) {
-    const int e = threadIdx.x + q * NT;
-    if (e < g.nbox * TX) {
-      const int ii = e / TX, c = e % TX, iky = box_row(ii, g);
-      const double ky = (double)ky_of(iky, g.n2);
-      lds[iky * S + c] = xn[q];
-      lds[iky * S + TX + c] = mul_iky(xn[q], ky);
-    }
-  }
-  for (int e = threadIdx.x; e < (N - g.nbox) * TX; e += NT) {
-    const int iky = g.nlo + e / TX, c = e % TX;
-    lds[iky * S + c] = zc();
-    lds[iky * S + TX + c] = zc();
-  }
-  __syncthreads();
-  if (kx0 == 0) {
-    for (int j = 1 + (int)threadIdx.x; j < N / 2; j += NT) {
-      lds[(N - j) * S] = cconj(lds[j * S]);
-      lds[(N - j) * S + TX] = cconj(lds[j * S + TX]);
-    }
-    __syncthreads();
-  }
-  fft_lds<N, true, S, NT>(lds, d.tw);
-  for (int e = threadIdx.x; e < N * TX; e += NT) {
-    const int i = e / TX, c = e % TX, kx = kx0 + c;
-    if (kx < g.KX) {
-      W1[pz + (size_t)i * g.sp + kx] = lds[i * S + c];
-      if (f > 0) W2[pz + (size_t)i * g.sp + kx] = lds[i * S + TX + c];
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// row_qw: c2r in x of BR, BI, their x and y derivatives, the feedback
-// products J = BRy BIx - BRx BIy and |B|^2 (derivatives.f90:967-1000), r2c in
-// x; I3 = F(J) - kx^2/4 F(|B|^2), I5 = F(|B|^2)
-// ---------------------------------------------------------------------------
-template <int N, int P>
-__global__ __launch_bounds__(NT) void k_row_qw(Dev d, double scale) {
-  constexpr int S1 = 3 * P, S2 = P, H = N / 2 + 1;
-  constexpr int PPT = (P * N + NT - 1) / NT;
-  __shared__ double2 lds[N * S1];
-  const Geo& g = d.g;
-  const size_t row0 = (size_t)blockIdx.x * P;
-  double2* const* I = d.I;
-  for (int e = threadIdx.x; e < P * H; e += NT) {
-    const int p = e / H, k = e % H;
-    const size_t off = (row0 + p) * g.sp + k;
-    double2 br = zc(), bry = zc(), bi = zc(), biy = zc();
-    if (k < g.KX) { br = I[2][off]; bry = I[3][off]; bi = I[4][off]; biy = I[5][off]; }
-    const double kk = (double)k;
-    put_pair<N>(lds, S1, k, 3 * p + 0, br, bi);
-    put_pair<N>(lds, S1, k, 3 * p + 1, mul_iky(br, kk), mul_iky(bi, kk));
-    put_pair<N>(lds, S1, k, 3 * p + 2, bry, biy);
-  }
-  __syncthreads();
-  fft_lds<N, true, S1, NT>(lds, d.tw);
-  double2 pr[PPT];
-#pragma unroll
-  for (int q = 0; q < PPT; ++q) {
-    const int e = threadIdx.x + q * NT;
-    if (e < P * N) {
-      const int p = e % P, n = e / P;
-      const double2 a = lds[n * S1 + 3 * p], b = lds[n * S1 + 3 * p + 1], c = lds[n * S1 + 3 * p + 2];
-      // a = (BR, BI), b = (BRx, BIx), c = (BRy, BIy)
-      pr[q] = make_double2(c.x * b.y - b.x * c.y, a.x * a.x + a.y * a.y);
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < PPT; ++q) {
-    const int e = threadIdx.x + q * NT;
-    if (e < P * N) {
-      const int p = e % P, n = e / P;
-      lds[n * S2 + p] = pr[q];
-    }
-  }
-  __syncthreads();
-  fft_lds<N, false, S2, NT>(lds, d.tw);
-  for (int e = threadIdx.x; e < P * g.KX; e += NT) {
-    const int p = e / g.KX, k = e % g.KX;
-    const size_t off = (row0 + p) * g.sp + k;
-    double2 FJ, FM;
-    split_pair<N>(lds, S2, k, p, FJ, FM);
-    const double c4 = 0.25 * (double)k * (double)k;
-    I[3][off] = cscale(make_double2(FJ.x - c4 * FM.x, FJ.y - c4 * FM.y), scale);
-    I[5][off] = cscale(FM, scale);
-  }
-}
-
-// ---------------------------------------------------------------------------
-// col_qw: r2c in y of the feedback rows, q^w = Ro W2F L (F(J) - kh2/4 F(|B|^2))
-// (derivatives.f90:1010-1027) and q* = q - q^w where L = 1
-// (main_waqg.f90:430-439)
-// ---------------------------------------------------------------------------
-template <int N, int TX>
-__global__ __launch_bounds__(NT) void k_col_qw(Dev d) {
-  constexpr int S = 2 * TX;
-  __shared__ double2 lds[N * S];
-  const Geo& g = d.g;
-  const int kx0 = blockIdx.x * TX, z = blockIdx.y;
-  const size_t pz = (size_t)z * g.n2 * g.sp;
-  for (int e = threadIdx.x; e < N * S; e += NT) {
-    const int i = e / S, s = e % S, c = s % TX, kx = kx0 + c;
-    double2 v = zc();
-    if (kx < g.KX) v = (s < TX ? d.I[3] : d.I[5])[pz + (size_t)i * g.sp + kx];
-    lds[i * S + s] = v;
-  }
-  __syncthreads();
-  fft_lds<N, false, S, NT>(lds, d.tw);
-  const size_t sz = (size_t)z * g.ikty * g.sp;
-  for (int e = threadIdx.x; e < g.nbox * TX; e += NT) {
-    const int ii = e / TX, c = e % TX, kx = kx0 + c, iky = box_row(ii, g);
-    if (kx < g.KX) {
-      const size_t idx = sz + (size_t)iky * g.sp + kx;
-      double2 out = zc();
-      if (lmask(kx, iky, g)) {
-        const double ky = (double)ky_of(iky, g.n2);
-        const double2 F1 = lds[iky * S + c], F2 = lds[iky * S + TX + c];
-        const double c4 = 0.25 * (ky * ky);
-        const double2 qw = make_double2(((F1.x - c4 * F2.x) * d.Ro) * d.W2F, ((F1.y - c4 * F2.y) * d.Ro) * d.W2F);
-        const double2 qv = d.q[idx];
-        out = csub(qv, qw);
-      }
-      d.qs[idx] = out;
-    }
   }
 }
 
@@ -835,28 +512,6 @@ struct L {
     hipLaunchKernelGGL((k_col_inv<N, C::TXI>), grid, dim3(NT), 0, st, d.g, d.tw, F, out, W, box, mul, mask);
     return hipGetLastError();
   }
-  static hipError_t row_nl(const Dev& d, hipStream_t st, int Win) {
-    dim3 grid(d.g.n2 * d.g.n3 / C::PR);
-    const double scale = 1.0 / ((double)d.g.n1 * (double)d.g.n2);
-    hipLaunchKernelGGL((k_row_nl<N, C::PR>), grid, dim3(NT), 0, st, d, Win, scale);
-    return hipGetLastError();
-  }
-  static hipError_t col_upd(const Dev& d, hipStream_t st, int euler, int has_dq) {
-    dim3 grid((d.g.KX + C::TXU - 1) / C::TXU, d.g.n3, 3);
-    hipLaunchKernelGGL((k_col_upd<N, C::TXU>), grid, dim3(NT), 0, st, d, euler, has_dq);
-    return hipGetLastError();
-  }
-  static hipError_t row_qw(const Dev& d, hipStream_t st) {
-    dim3 grid(d.g.n2 * d.g.n3 / C::PR);
-    const double scale = 1.0 / ((double)d.g.n1 * (double)d.g.n2);
-    hipLaunchKernelGGL((k_row_qw<N, C::PR>), grid, dim3(NT), 0, st, d, scale);
-    return hipGetLastError();
-  }
-  static hipError_t col_qw(const Dev& d, hipStream_t st) {
-    dim3 grid((d.g.KX + C::TXU - 1) / C::TXU, d.g.n3);
-    hipLaunchKernelGGL((k_col_qw<N, C::TXU>), grid, dim3(NT), 0, st, d);
-    return hipGetLastError();
-  }
   static hipError_t r2c(const Dev& d, hipStream_t st, const double* r, int ld, double2* k, int nz) {
     const double scale = 1.0 / ((double)d.g.n1 * (double)d.g.n2);
     hipLaunchKernelGGL((k_row_r2c<N, C::PRC>), dim3(d.g.n2 * nz / C::PRC), dim3(NT), 0, st, d.g, d.tw, r, ld,
@@ -888,16 +543,6 @@ struct L {
 
 bool supported_n(int n) { return n == 16 || n == 32 || n == 64 || n == 128 || n == 256 || n == 512; }
 
-hipError_t launch_col_inv(const Dev& d, hipStream_t st, const double2* F, double2* out, int W, int box, int mul,
-                          int mask, int nz) {
-  QG_DISPATCH(d.g.n2, col_inv(d, st, F, out, W, box, mul, mask, nz))
-}
-hipError_t launch_row_nl(const Dev& d, hipStream_t st, int Win) { QG_DISPATCH(d.g.n1, row_nl(d, st, Win)) }
-hipError_t launch_col_upd(const Dev& d, hipStream_t st, int euler, int has_dq) {
-  QG_DISPATCH(d.g.n2, col_upd(d, st, euler, has_dq))
-}
-hipError_t launch_row_qw(const Dev& d, hipStream_t st) { QG_DISPATCH(d.g.n1, row_qw(d, st)) }
-hipError_t launch_col_qw(const Dev& d, hipStream_t st) { QG_DISPATCH(d.g.n2, col_qw(d, st)) }
 hipError_t launch_r2c(const Dev& d, hipStream_t st, const double* r, int ld, double2* k, int nz) {
   QG_DISPATCH(d.g.n1, r2c(d, st, r, ld, k, nz))
 }
@@ -958,7 +603,7 @@ hipError_t launch_diag(const Dev& d, hipStream_t st, int* nblocks) {
 }
 
 const char* kernel_name(int id) {
-  static const char* names[K_NKERNELS] = {"col_inv", "row_nl", "col_upd", "row_qw", "col_qw", "solve", "aux"};
+  static const char* names[K_NKERNELS] = {"col_inv", "row_nl", "col_upd", "row_qw", "col_qw", "solve", "solve_a", "aux"};
   return (id >= 0 && id < K_NKERNELS) ? names[id] : "?";
 }
 

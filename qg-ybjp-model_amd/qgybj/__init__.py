@@ -22,13 +22,13 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libqgybj.so")
 
 QG_OK = 0
 ERRORS = {1: "QG_ERR_ARG", 2: "QG_ERR_HIP", 3: "QG_ERR_NOMEM", 4: "QG_ERR_NONFINITE", 5: "QG_ERR_STATE"}
-KERNEL_NAMES = ["col_inv", "row_nl", "col_upd", "row_qw", "col_qw", "solve", "aux"]
+KERNEL_NAMES = ["col_inv", "row_nl", "col_upd", "row_qw", "col_qw", "solve", "solve_a", "aux"]
 
 # exported symbols declared in include/qgybj.h
 SYMBOLS = [
     "qg_create", "qg_destroy", "qg_last_error", "qg_set_state", "qg_set_state_physical", "qg_get_state",
     "qg_step", "qg_iter", "qg_diag", "qg_init_q", "qg_fft_r2c", "qg_fft_c2r", "qg_device_field", "qg_device_pitch", "qg_stream",
-    "qg_sync", "qg_timing", "qg_kernel_times", "qg_kernel_name", "qg_mask", "qg_box", "qg_version",
+    "qg_sync", "qg_timing", "qg_kernel_times", "qg_kernel_name", "qg_set_variants", "qg_get_variants", "qg_mask", "qg_box", "qg_version",
 ]
 
 
@@ -82,6 +82,8 @@ def load(path: str = LIB_PATH):
     lib.qg_kernel_times.argtypes = [vp, dp, C.POINTER(C.c_int32), C.c_int32]
     lib.qg_kernel_name.argtypes = [C.c_int32]
     lib.qg_kernel_name.restype = C.c_char_p
+    lib.qg_set_variants.argtypes = [vp, C.POINTER(C.c_int32), C.c_int32]
+    lib.qg_get_variants.argtypes = [vp, C.POINTER(C.c_int32), C.c_int32]
     lib.qg_mask.argtypes = [C.c_int32, C.c_int32, C.POINTER(C.c_int32)]
     lib.qg_box.argtypes = [C.c_int32, C.c_int32, C.POINTER(C.c_int32)]
     lib.qg_version.restype = C.c_char_p
@@ -211,6 +213,16 @@ class Member:
         self._chk(self.lib.qg_fft_c2r(self._h, _dp(k), _dp(r), ldx, nz), "qg_fft_c2r")
         return r[..., : self.n1]
 
+    def set_variants(self, e16):
+        a = np.ascontiguousarray(e16, dtype=np.int32)
+        self._chk(self.lib.qg_set_variants(self._h, a.ctypes.data_as(C.POINTER(C.c_int32)), len(a)),
+                  "qg_set_variants")
+
+    def get_variants(self):
+        a = np.zeros(5, dtype=np.int32)
+        self._chk(self.lib.qg_get_variants(self._h, a.ctypes.data_as(C.POINTER(C.c_int32)), 5), "qg_get_variants")
+        return a
+
     def sync(self):
         self._chk(self.lib.qg_sync(self._h), "qg_sync")
 
@@ -218,8 +230,9 @@ class Member:
         self._chk(self.lib.qg_timing(self._h, 1 if on else 0), "qg_timing")
 
     def kernel_times(self):
-        ms = np.zeros(7)
-        cnt = np.zeros(7, dtype=np.int32)
-        self._chk(self.lib.qg_kernel_times(self._h, _dp(ms), cnt.ctypes.data_as(C.POINTER(C.c_int32)), 7),
+        n = len(KERNEL_NAMES)
+        ms = np.zeros(n)
+        cnt = np.zeros(n, dtype=np.int32)
+        self._chk(self.lib.qg_kernel_times(self._h, _dp(ms), cnt.ctypes.data_as(C.POINTER(C.c_int32)), n),
                   "qg_kernel_times")
-        return {KERNEL_NAMES[i]: (float(ms[i]), int(cnt[i])) for i in range(7)}
+        return {KERNEL_NAMES[i]: (float(ms[i]), int(cnt[i])) for i in range(n)}

@@ -198,3 +198,23 @@ def test_vertical_partitions(n3):
     P = make_params(16, n3, W2F=0.05, dt=2e-3, stratification="exponential")
     m, S, g, OP, worst = _run_pair(P, 3, k0=2, check_every=True)
     m.close()
+
+
+@pytest.mark.parametrize("n,n3", [(32, 4), (64, 8), (256, 4), (512, 2)])
+@pytest.mark.parametrize("variant", [(0, 0, 0, 0, 0), (1, 1, 1, 1, 1), (0, 2, 0, 2, 0), (1, 3, 1, 3, 1)])
+def test_kernel_variants(n, n3, variant):
+    """Every transform-kernel variant the tuner can select gives the oracle's
+    result."""
+    P = make_params(n, n3, W2F=0.05, dt=2e-3)
+    g = O.Grid(n, n, n3)
+    OP = oracle_params(P)
+    psi, BR, BI = physical_ics(P, k0=3)
+    S = O.initial_state(g, OP, psi, BR, BI)
+    m = _member(P)
+    m.set_variants(variant)
+    assert list(m.get_variants()) == list(variant)
+    m.set_state_physical(psi, BR, BI)
+    S = O.run(g, OP, S, 3)
+    m.step(3)
+    compare_state(m.get_state(), S, 1e-11)
+    m.close()
