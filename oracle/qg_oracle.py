@@ -133,14 +133,19 @@ def n2_profile(z: np.ndarray, kind: str, ratio: float = 100.0) -> np.ndarray:
     if kind == "constant":
         return np.ones_like(z)
     if kind == "skewed_gaussian":
-        N02_sg = 0.537713935783168
-        N12_sg = 2.684198470106461
-        sigma_sg = 0.648457170048730
-        z0_sg = 6.121537923499139
-        alpha_sg = -5.338431587899242
+        # the constants are default-REAL literals assigned to double
+        # precision parameters, and sqrt(2.) is a REAL(4) square root: all
+        # enter the FP64 expression with float32 values
+        f32 = lambda x: float(np.float32(x))  # noqa: E731
+        N02_sg = f32(0.537713935783168)
+        N12_sg = f32(2.684198470106461)
+        sigma_sg = f32(0.648457170048730)
+        z0_sg = f32(6.121537923499139)
+        alpha_sg = f32(-5.338431587899242)
+        sqrt2 = float(np.sqrt(np.float32(2.0)))
         from scipy.special import erf
         return (N12_sg * np.exp(-((z - z0_sg) ** 2) / (sigma_sg ** 2))
-                * (1.0 + erf(alpha_sg * (z - z0_sg) / (sigma_sg * math.sqrt(2.0))))
+                * (1.0 + erf(alpha_sg * (z - z0_sg) / (sigma_sg * sqrt2)))
                 + N02_sg)
     if kind == "exponential":
         return np.exp(math.log(ratio) * (z - TWOPI) / TWOPI)

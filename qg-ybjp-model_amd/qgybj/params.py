@@ -100,11 +100,19 @@ def int_factor(n1: int, delt: float, nu1: float, ilap1: int, nu2: float, ilap2: 
 
 
 def n2_profile(z, kind: str, ratio: float = 100.0):
-    """Nondimensional N^2(z): 'constant' (init.f90:186-188), 'exponential'
+    """Nondimensional N^2(z): 'constant' (init.f90:186-188), 'skewed_gaussian'
+    (init.f90:189-191 with the default-REAL constants of
+    parameters_test1.f90:58-63 and the REAL(4) sqrt(2.)), 'exponential'
     (surface value 1 at z = L3, bottom value 1/ratio at z = 0)."""
     z = np.asarray(z, dtype=np.float64)
     if kind == "constant":
         return np.ones_like(z)
+    if kind == "skewed_gaussian":
+        from scipy.special import erf
+        n02, n12, sig, z0, alpha = (_f32(c) for c in (0.537713935783168, 2.684198470106461, 0.648457170048730,
+                                                       6.121537923499139, -5.338431587899242))
+        sqrt2 = _f32(np.sqrt(np.float32(2.0)))
+        return n12 * np.exp(-((z - z0) ** 2) / (sig ** 2)) * (1.0 + erf(alpha * (z - z0) / (sig * sqrt2))) + n02
     if kind == "exponential":
         return np.exp(math.log(ratio) * (z - TWOPI) / TWOPI)
     raise ValueError(kind)

@@ -1,0 +1,106 @@
+"""The Fortran driver (qg-ybjp-model_amd/fortran) over the C ABI.
+
+CPU: the driver's parameter derivations (nuh, nuz in the reference's REAL(4)
+expressions, parameters_test1.f90:106-113) agree bit for bit with the ones the
+Python harness passes, so both hosts hand the library the same tables.
+
+GPU: a run of the driver from raw-file initial conditions ends in the same
+state as the oracle (10 steps, <= 1e-11 relative L2 per field), and its
+printed energies match the oracle's diagnostics.
+"""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import PKG
+from helpers import FIELDS, compare_state, oracle_params, physical_ics
+from oracle import qg_oracle as O
+from qgybj.params import make_params, nuh_coeff
+
+DRIVER = os.path.join(PKG, "lib", "qg_driver")
+
+pytestmark = pytest.mark.skipif(shutil.which("amdflang") is None and not os.path.exists(DRIVER),
+                                reason="no Fortran compiler and no prebuilt driver")
+
+
+def _nml(path, **kv):
+    lines = ["&qg"]
+    for k, v in kv.items():
+        if isinstance(v, str):
+            lines.append(f"  {k} = '{v}'")
+        elif isinstance(v, float):
+            lines.append(f"  {k} = " + f"{v:.17e}".replace("e", "d"))
+        else:
+            lines.append(f"  {k} = {v}")
+    lines.append("/")
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def _driver():
+    if not os.path.exists(DRIVER):
+        import build
+        build.build_driver()
+    assert os.path.exists(DRIVER)
+    return DRIVER
+
+
+def _run(tmp_path, **kv):
+    nml = os.path.join(tmp_path, "run.nml")
+    _nml(nml, **kv)
+    return subprocess.run([_driver(), nml], capture_output=True, text=True, cwd=tmp_path, timeout=300)
+
+
+@pytest.mark.parametrize("n1,c1,l1,c2,l2", [(64, 0.0, 4, 10.0, 4), (32, 0.01, 2, 10.0, 6),
+                                             (128, 1.0, 1, 3.0, 3), (256, 0.0, 4, 10.0, 4)])
+def test_driver_parameters_match_harness(tmp_path, n1, c1, l1, c2, l2):
+    r = _run(tmp_path, n1=n1, n3=8, coeff1=c1, ilap1=l1, coeff2=c2, ilap2=l2, coeff1w=c2, ilap1w=l2,
+             coeff2w=c1, ilap2w=l1, coeffz=0.5, itermax=0)
+    head = [ln for ln in r.stdout.splitlines() if ln.startswith("# nuh1")]
+    assert head, r.stdout + r.stderr
+    vals = [float(x) for x in head[0].split()[6:]]
+    want = [nuh_coeff(c1, n1, l1), nuh_coeff(c2, n1, l2), nuh_coeff(c2, n1, l2), nuh_coeff(c1, n1, l1),
+            make_params(n1, 8, coeffz=0.5).nuz]
+    # 17 significant digits printed: equal as FP64 numbers
+    for got, w in zip(vals, want):
+        assert got == w or abs(got - w) <= 1e-16 * abs(w), (vals, want)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n1,n3,flags", [(32, 16, 0), (64, 8, 2)])
+def test_driver_run_matches_oracle(tmp_path, n1, n3, flags):
+    kw = dict(dt=2e-3, W2F=0.05, flags=flags)
+    P = make_params(n1, n3, **kw)
+    psi, BR, BI = physical_ics(P, k0=3, seed=2)
+    for name, a in (("psi", psi), ("BR", BR), ("BI", BI)):
+        np.ascontiguousarray(a, dtype=np.float64).tofile(os.path.join(tmp_path, f"ic_{name}.in"))
+    nsteps = 10
+    print(f"fortran driver n1={n1} n3={n3} flags={flags}", flush=True)
+    r = _run(tmp_path, n1=n1, n3=n3, delt=2e-3, W2F=0.05, no_feedback=1 if flags & 2 else 0, itermax=nsteps,
+             diag_every=5, ic_prefix=os.path.join(tmp_path, "ic_"), out_prefix=os.path.join(tmp_path, "out_"))
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert f"# done iter {nsteps}" in r.stdout
+
+    iktx = n1 // 2 + 1
+    raw = np.fromfile(os.path.join(tmp_path, "out_state.bin"), dtype=np.complex128)
+    raw = raw.reshape(len(FIELDS), n3, n1, iktx)
+    got = {f: raw[i] for i, f in enumerate(FIELDS)}
+
+    g = O.Grid(n1, n1, n3)
+    OP = oracle_params(P)
+    S = O.initial_state(g, OP, psi, BR, BI)
+    diags = {0: O.diagnostics(g, OP, S)}
+    for it in range(1, nsteps + 1):
+        S = O.step(g, OP, S)
+        if it % 5 == 0:
+            diags[it] = O.diagnostics(g, OP, S)
+    compare_state(got, S, 1e-11)
+
+    rows = [ln.split() for ln in r.stdout.splitlines() if ln and not ln.startswith("#")]
+    seen = {int(w[0]): np.array([float(x) for x in w[2:]]) for w in rows}
+    assert sorted(seen) == [0, 5, 10]
+    for it, d in diags.items():
+        np.testing.assert_allclose(seen[it], d, rtol=1e-11, atol=1e-14 * np.max(np.abs(d)))
