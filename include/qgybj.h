@@ -132,21 +132,29 @@ int qg_device_pitch(const qg_ctx* ctx, int32_t* pitch);
 int qg_stream(qg_ctx* ctx, void** stream);
 int qg_sync(qg_ctx* ctx);
 
-/* Per-kernel timing with HIP events on the context's stream: enable, then
- * read accumulated milliseconds and launch counts per kernel id
- * (0 col_inv, 1 row_nl, 2 col_upd, 3 row_qw, 4 col_qw, 5 solve (psi), 6 solve_a
- * (A, on the side stream), 7 aux).
- * Reading resets the accumulators. */
+/* Per-kernel timing with HIP events on the stream each kernel runs on:
+ * enable, then read accumulated milliseconds and launch counts per kernel id
+ * (0 col_inv, 1 row_nl, 2 col_upd, 3 row_qw, 4 col_qw, 5 solve (psi, and A
+ * when the A inversions run on the member's stream), 6 solve_a (A alone),
+ * 7 aux).  Reading resets the accumulators. */
 int qg_timing(qg_ctx* ctx, int32_t enable);
 int qg_kernel_times(qg_ctx* ctx, double* ms, int32_t* counts, int32_t n);
 const char* qg_kernel_name(int32_t id);
 
-/* Kernel variants of the five transform kernels (ps_inv, row_nl, col_upd,
- * row_qw, col_qw).  Column kernels: 0 = 8 elements per thread, 1 = 16.  Row
- * kernels: 0 = 8 per thread in 256-thread blocks, 1 = 16/256, 2 = 8/64,
- * 3 = 16/64.  Set by the tuner; results agree to FP64 rounding. */
-int qg_set_variants(qg_ctx* ctx, const int32_t* e16, int32_t n);
-int qg_get_variants(const qg_ctx* ctx, int32_t* e16, int32_t n);
+/* Kernel variants, v[0..n-1] with n <= QG_NVARIANTS:
+ *  v[0..4] the five transform kernels (ps_inv, row_nl, col_upd, row_qw,
+ *          col_qw).  Column kernels: 0 = 8 elements per thread, 1 = 16.  Row
+ *          kernels: 0 = 8 per thread in 256-thread blocks, 1 = 16/256,
+ *          2 = 8/64, 3 = 16/64.
+ *  v[5]    shape of the partitioned vertical solve: 0..3 = (levels per
+ *          chunk, columns per block) of (8,16), (8,32), (16,16), (16,32).
+ *  v[6]    1: the A inversions run in the psi inversion's launch on the
+ *          member's stream; 0: on a side stream, overlapped with q^w.
+ * Set by the tuner; results agree to FP64 rounding.  Out-of-range values are
+ * rejected (QG_ERR_ARG) and leave the settings unchanged. */
+enum { QG_NVARIANTS = 7 };
+int qg_set_variants(qg_ctx* ctx, const int32_t* v, int32_t n);
+int qg_get_variants(const qg_ctx* ctx, int32_t* v, int32_t n);
 
 /* Host-only helpers (no GPU needed): the dealiasing mask L(iktx, ikty) as the
  * library evaluates it, and the box geometry {kmax, KX, nlo, nhi}. */

@@ -201,7 +201,9 @@ int step_once(qg_ctx* c) {
   HIPCHK(c, hipStreamWaitEvent(st, c->evA, 0));
   KRUN(c, qg::K_COL_UPD, qg::launch_col_upd(d, st, euler ? 1 : 0, has_dq));
   c->p_valid = 1;
-  if (!(flags & qg::F_PASSIVE)) {
+  const bool solve_a = !(flags & qg::F_PASSIVE);
+  const bool a_side = solve_a && !d.tune.a_main;
+  if (a_side) {
     // A^{n+1} from B^{n+1} (A_solver_ybj_plus) on the side stream; it is
     // needed by the next update only
     HIPCHK(c, hipEventRecord(c->evB, st));
@@ -209,6 +211,10 @@ int step_once(qg_ctx* c) {
     KRUN_ON(c, c->stream2, qg::K_SOLVE_A, qg::launch_solve(d, c->stream2, 1, 2, nullptr));
     HIPCHK(c, hipEventRecord(c->evA, c->stream2));
   }
+  // the inversions left for the member's stream: psi (field 0) unless the
+  // flow is fixed, then AR, AI (fields 1, 2) unless they ran on the side
+  const int f_lo = (flags & qg::F_FIXED_FLOW) ? 1 : 0;
+  const int f_hi = (solve_a && !a_side) ? 2 : 0;
   if (!(flags & qg::F_FIXED_FLOW)) {
     const double2* rhs = d.q;
     if (!(flags & qg::F_NO_FEEDBACK)) {
@@ -216,7 +222,9 @@ int step_once(qg_ctx* c) {
       KRUN(c, qg::K_COL_QW, qg::launch_col_qw(d, st));
       rhs = d.qs;
     }
-    KRUN(c, qg::K_SOLVE, qg::launch_solve(d, st, 0, 0, rhs));
+    KRUN(c, qg::K_SOLVE, qg::launch_solve(d, st, f_lo, f_hi, rhs));
+  } else if (f_hi > 0) {
+    KRUN(c, qg::K_SOLVE_A, qg::launch_solve(d, st, f_lo, f_hi, nullptr));
   }
   if (c->clear_pending > 0) {
     HIPCHK(c, hipStreamWaitEvent(st, c->evA, 0));
@@ -318,6 +326,10 @@ int qg_create(const qg_params* p, qg_ctx** out) {
   Dev& d = c->d;
   d.g = qg::make_geo(p->n1, p->n2, p->n3);
   const Geo& g = d.g;
+  // measured default at 256^2 x 128: 16-level chunks, 32 columns per block,
+  // A inversions on the side stream
+  d.tune.solve = 3;
+  d.tune.a_main = 0;
   d.flags = p->flags;
   d.dt = p->dt;
   d.gamma = p->gamma;
@@ -573,15 +585,26 @@ int qg_device_field(qg_ctx* c, int32_t which, void** dptr) {
   return QG_OK;
 }
 
-int qg_set_variants(qg_ctx* c, const int32_t* e16, int32_t n) {
-  if (!c || !e16 || n < 0) return QG_ERR_ARG;
-  for (int i = 0; i < n && i < 5; ++i) c->d.tune.e16[i] = (e16[i] >= 0 && e16[i] <= 3) ? e16[i] : 0;
+int qg_set_variants(qg_ctx* c, const int32_t* v, int32_t n) {
+  if (!c || !v || n < 0) return QG_ERR_ARG;
+  for (int i = 0; i < n && i < QG_NVARIANTS; ++i) {
+    const int hi = i < 6 ? 3 : 1;
+    if (v[i] < 0 || v[i] > hi) return fail(c, QG_ERR_ARG, "qg_set_variants: value out of range");
+  }
+  // the A inversions change stream: let any in-flight side-stream work end
+  if (n > 6) HIPCHK(c, hipStreamWaitEvent(c->stream, c->evA, 0));
+  for (int i = 0; i < n && i < QG_NVARIANTS; ++i) {
+    if (i < 5) c->d.tune.e16[i] = v[i];
+    else if (i == 5) c->d.tune.solve = v[i];
+    else c->d.tune.a_main = v[i];
+  }
   return QG_OK;
 }
 
-int qg_get_variants(const qg_ctx* c, int32_t* e16, int32_t n) {
-  if (!c || !e16 || n < 0) return QG_ERR_ARG;
-  for (int i = 0; i < n && i < 5; ++i) e16[i] = c->d.tune.e16[i];
+int qg_get_variants(const qg_ctx* c, int32_t* v, int32_t n) {
+  if (!c || !v || n < 0) return QG_ERR_ARG;
+  for (int i = 0; i < n && i < QG_NVARIANTS; ++i)
+    v[i] = i < 5 ? c->d.tune.e16[i] : (i == 5 ? c->d.tune.solve : c->d.tune.a_main);
   return QG_OK;
 }
 

@@ -56,8 +56,13 @@ enum : int {
 // Kernel variant choice per hot kernel (ps_inv, row_nl, col_upd, row_qw,
 // col_qw).  Column kernels: 1 = 16 elements per thread, 0 = 8.  Row kernels:
 // 0 = 8 elements / 256 threads, 1 = 16 / 256, 2 = 8 / 64, 3 = 16 / 64.
+// solve: shape of the partitioned vertical solve (solve_shape).  a_main: 1 =
+// the A inversions run with the psi inversion in one launch on the member's
+// stream, 0 = on a side stream overlapped with the q^w work.
 struct Tune {
   int e16[5];
+  int solve;
+  int a_main;
 };
 
 // Device-side view of one member: state, work buffers and tables.
@@ -98,6 +103,7 @@ hipError_t launch_col_upd(const Dev& d, hipStream_t st, int euler, int has_dq);
 hipError_t launch_row_qw(const Dev& d, hipStream_t st);
 hipError_t launch_col_qw(const Dev& d, hipStream_t st);
 hipError_t launch_solve(const Dev& d, hipStream_t st, int f_lo, int f_hi, const double2* rhs_psi);
+void solve_shape(int n3, int v, int* M, int* P, int* LPC);
 hipError_t launch_dq(const Dev& d, hipStream_t st);
 hipError_t launch_clear_outside(const Dev& d, hipStream_t st, double2* f, int all);
 hipError_t launch_init_q(const Dev& d, hipStream_t st, const double2* psi, double2* q);

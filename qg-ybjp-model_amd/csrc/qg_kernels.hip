@@ -142,24 +142,24 @@ __global__ __launch_bounds__(NT) void k_solve(Dev d, int f_lo, const double2* __
 }
 
 // ---------------------------------------------------------------------------
-// solve_p: the same systems with the z range of every column split into P
-// chunks of M levels (one wave per chunk, 32 columns x {Re, Im} per wave).
-// Each chunk eliminates its own block in registers (DGTSV order, no
-// interchanges) for three right-hand sides: its data g and the two coupling
-// spikes v (from the row above the chunk) and w (from the row below), so that
-// x = g - x_above * v - x_below * w.  The 2P boundary unknowns of a column form
-// a block-tridiagonal system with 2x2 blocks, solved by one lane per column
-// in LDS; each chunk then reconstructs its levels.  Padding rows beyond n3
-// are identity rows decoupled from the system.
+// solve_c: the same systems with the z range of every column split into P
+// chunks of M levels; LPC columns per block, one lane per complex column (the
+// matrix is real, so Re and Im share one factorisation).  Each chunk
+// eliminates its own block in registers (DGTSV order, no interchanges) for
+// three right-hand sides: its data g and the two coupling spikes v (from the
+// row above the chunk) and w (from the row below), so that
+// x = g - x_above * v - x_below * w.  The 2P boundary unknowns of a column
+// form a block-tridiagonal system with 2x2 blocks, solved by the lanes of
+// chunk 0 in LDS; each chunk then reconstructs its levels.  Padding rows
+// beyond n3 are identity rows decoupled from the system.
 // ---------------------------------------------------------------------------
 template <int M, int P, int LPC>
-__global__ __launch_bounds__(LPC * P) void k_solve_p(Dev d, int f_lo, const double2* __restrict__ rhs_psi) {
+__global__ __launch_bounds__(LPC * P) void k_solve_c(Dev d, int f_lo, const double2* __restrict__ rhs_psi) {
   const Geo& g = d.g;
-  // LPC lanes per chunk: LPC/2 columns x {Re, Im}
   const int lane = threadIdx.x % LPC, c = threadIdx.x / LPC;
   const int f = f_lo + blockIdx.y;
   const int ncols = g.KX * g.nbox;
-  const int col = blockIdx.x * (LPC / 2) + (lane >> 1), part = lane & 1;
+  const int col = blockIdx.x * LPC + lane;
   const bool valid = col < ncols;
   int kx = 0, iky = 0, kh2 = 0;
   if (valid) {
@@ -169,12 +169,10 @@ __global__ __launch_bounds__(LPC * P) void k_solve_p(Dev d, int f_lo, const doub
     kh2 = kx * kx + ky * ky;
   }
   const bool active = valid && kh2 != 0 && lmask(kx, iky, g);
-  const double2* rhs = f == 0 ? rhs_psi : (f == 1 ? d.BR : d.BI);
-  double2* out = f == 0 ? d.psi : (f == 1 ? d.AR : d.AI);
-  const double* __restrict__ rin = reinterpret_cast<const double*>(rhs) + part;
-  double* __restrict__ xo = reinterpret_cast<double*>(out) + part;
-  const size_t plane2 = (size_t)2 * g.ikty * g.sp;
-  const size_t c2 = (size_t)2 * ((size_t)iky * g.sp + kx);
+  const double2* __restrict__ rhs = f == 0 ? rhs_psi : (f == 1 ? d.BR : d.BI);
+  double2* __restrict__ out = f == 0 ? d.psi : (f == 1 ? d.AR : d.AI);
+  const size_t plane = (size_t)g.ikty * g.sp;
+  const size_t ci = (size_t)iky * g.sp + kx;
   const int n3 = g.n3;
   const double dz = d.dz;
   // inactive columns get a regular (kh2 = 1) system so that no lane divides
@@ -184,106 +182,132 @@ __global__ __launch_bounds__(LPC * P) void k_solve_p(Dev d, int f_lo, const doub
   const double rs = f == 0 ? dz * dz : (dz * dz) * d.Bu;
   const int s = c * M;
 
-  // local elimination for g (data), v (top spike), w (bottom spike)
-  double dp[M], gv[M], vv[M], wv[M];
+  double2 gv[M];
+  double rp[M], vv[M], wv[M];
 #pragma unroll
   for (int i = 0; i < M; ++i) {
     const int z = s + i;
-    gv[i] = (active && z < n3) ? rs * rin[z * plane2 + c2] : 0.0;
+    double2 b = make_double2(0.0, 0.0);
+    if (active && z < n3) b = rhs[z * plane + ci];
+    gv[i] = make_double2(rs * b.x, rs * b.y);
   }
-  const double ltop = (s > 0 && s < n3) ? d.dl[s - 1] : 0.0;          // coupling of row s to row s-1
+  const double ltop = (s > 0 && s < n3) ? d.dl[s - 1] : 0.0;  // coupling of row s to row s-1
   const int e = s + M - 1;
-  const double ubot = (e < n3 - 1) ? d.du[e] : 0.0;                  // coupling of row e to row e+1
+  const double ubot = (e < n3 - 1) ? d.du[e] : 0.0;          // coupling of row e to row e+1
+  // forward elimination; rp holds the reciprocal pivots
 #pragma unroll
   for (int i = 0; i < M; ++i) {
     const int z = s + i;
-    const double dz_ = z < n3 ? -(d.dsum[z] + kt) : 1.0;
+    const double dd = z < n3 ? -(d.dsum[z] + kt) : 1.0;
     if (i == 0) {
-      dp[0] = dz_;
+      rp[0] = 1.0 / dd;
       vv[0] = ltop;
-      wv[0] = (M == 1) ? ubot : 0.0;
     } else {
       const double lz = z < n3 ? d.dl[z - 1] : 0.0;
       const double uz = (z - 1 < n3 - 1) ? d.du[z - 1] : 0.0;
-      const double fact = lz / dp[i - 1];
-      dp[i] = dz_ - fact * uz;
-      gv[i] = gv[i] - fact * gv[i - 1];
+      const double fact = lz * rp[i - 1];
+      rp[i] = 1.0 / (dd - fact * uz);
+      gv[i].x -= fact * gv[i - 1].x;
+      gv[i].y -= fact * gv[i - 1].y;
       vv[i] = -fact * vv[i - 1];
-      wv[i] = (i == M - 1) ? ubot : 0.0;
     }
+    wv[i] = (i == M - 1) ? ubot : 0.0;
   }
-  {
-    const double inv = 1.0 / dp[M - 1];
-    gv[M - 1] *= inv;
-    vv[M - 1] *= inv;
-    wv[M - 1] *= inv;
-  }
+  gv[M - 1].x *= rp[M - 1];
+  gv[M - 1].y *= rp[M - 1];
+  vv[M - 1] *= rp[M - 1];
+  wv[M - 1] *= rp[M - 1];
 #pragma unroll
   for (int i = M - 2; i >= 0; --i) {
     const int z = s + i;
     const double uz = z < n3 - 1 ? d.du[z] : 0.0;
-    const double inv = 1.0 / dp[i];
-    gv[i] = (gv[i] - uz * gv[i + 1]) * inv;
-    vv[i] = (vv[i] - uz * vv[i + 1]) * inv;
-    wv[i] = (wv[i] - uz * wv[i + 1]) * inv;
+    gv[i].x = (gv[i].x - uz * gv[i + 1].x) * rp[i];
+    gv[i].y = (gv[i].y - uz * gv[i + 1].y) * rp[i];
+    vv[i] = (vv[i] - uz * vv[i + 1]) * rp[i];
+    wv[i] = (wv[i] - uz * wv[i + 1]) * rp[i];
   }
   // reduced system: y_c = (T_c, B_c) = (x_s, x_e);
   //   T_c + v_s B_{c-1} + w_s T_{c+1} = g_s ;  B_c + v_e B_{c-1} + w_e T_{c+1} = g_e
-  __shared__ double red[P][6][LPC];
-  red[c][0][lane] = gv[0];
-  red[c][1][lane] = gv[M - 1];
-  red[c][2][lane] = vv[0];
-  red[c][3][lane] = vv[M - 1];
-  red[c][4][lane] = wv[0];
-  red[c][5][lane] = wv[M - 1];
-  __syncthreads();
-  if (c == 0) {
-    // Block Thomas with A_b = [[0,vs],[0,ve]], C_b = [[ws,0],[we,0]].  Every
-    // eliminated diagonal block has the form [[D0,0],[D2,1]]; (h0, h1, D0, D2)
-    // of block b overwrite (g_s, g_e, v_s, v_e) of chunk b in LDS.
-    double D0p = 1.0, D2p = 0.0;
-    double h0p = red[0][0][lane], h1p = red[0][1][lane];
-    red[0][2][lane] = 1.0;
-    red[0][3][lane] = 0.0;
+  // slots: 0,1 g_s (re, im); 2,3 g_e; 4 v_s; 5 v_e; 6 w_s; 7 w_e
+  __shared__ double red[P][8][LPC];
+  if constexpr (P > 1) {
+    red[c][0][lane] = gv[0].x;
+    red[c][1][lane] = gv[0].y;
+    red[c][2][lane] = gv[M - 1].x;
+    red[c][3][lane] = gv[M - 1].y;
+    red[c][4][lane] = vv[0];
+    red[c][5][lane] = vv[M - 1];
+    red[c][6][lane] = wv[0];
+    red[c][7][lane] = wv[M - 1];
+    __syncthreads();
+    if (c == 0) {
+      // Block Thomas with A_b = [[0,vs],[0,ve]], C_b = [[ws,0],[we,0]].  Every
+      // eliminated diagonal block has the form [[D0,0],[D2,1]]; (h0, h1,
+      // 1/D0, D2) of block b overwrite (g_s, g_e, v_s, v_e) of chunk b.
+      double rD0p = 1.0, D2p = 0.0;
+      double2 h0p = make_double2(red[0][0][lane], red[0][1][lane]);
+      double2 h1p = make_double2(red[0][2][lane], red[0][3][lane]);
+      red[0][4][lane] = 1.0;
+      red[0][5][lane] = 0.0;
 #pragma unroll 1
-    for (int b = 1; b < P; ++b) {
-      const double i10 = -D2p / D0p;  // second row of the inverse: (i10, 1)
-      const double vs = red[b][2][lane], ve = red[b][3][lane];
-      const double ws = red[b - 1][4][lane], we = red[b - 1][5][lane];
-      const double m00 = vs * i10, m01 = vs, m10 = ve * i10, m11 = ve;
-      const double D0 = 1.0 - (m00 * ws + m01 * we);
-      const double D2 = -(m10 * ws + m11 * we);
-      const double h0 = red[b][0][lane] - (m00 * h0p + m01 * h1p);
-      const double h1 = red[b][1][lane] - (m10 * h0p + m11 * h1p);
-      red[b][0][lane] = h0;
-      red[b][1][lane] = h1;
-      red[b][2][lane] = D0;
-      red[b][3][lane] = D2;
-      D0p = D0; D2p = D2; h0p = h0; h1p = h1;
-    }
-    double yT = 0.0;  // T of block b+1
-#pragma unroll 1
-    for (int b = P - 1; b >= 0; --b) {
-      double t0 = red[b][0][lane], t1 = red[b][1][lane];
-      if (b < P - 1) {
-        t0 -= red[b][4][lane] * yT;
-        t1 -= red[b][5][lane] * yT;
+      for (int b = 1; b < P; ++b) {
+        const double i10 = -D2p * rD0p;  // second row of the inverse: (i10, 1)
+        const double vs = red[b][4][lane], ve = red[b][5][lane];
+        const double ws = red[b - 1][6][lane], we = red[b - 1][7][lane];
+        const double m00 = vs * i10, m01 = vs, m10 = ve * i10, m11 = ve;
+        const double rD0 = 1.0 / (1.0 - (m00 * ws + m01 * we));
+        const double D2 = -(m10 * ws + m11 * we);
+        double2 h0, h1;
+        h0.x = red[b][0][lane] - (m00 * h0p.x + m01 * h1p.x);
+        h0.y = red[b][1][lane] - (m00 * h0p.y + m01 * h1p.y);
+        h1.x = red[b][2][lane] - (m10 * h0p.x + m11 * h1p.x);
+        h1.y = red[b][3][lane] - (m10 * h0p.y + m11 * h1p.y);
+        red[b][0][lane] = h0.x;
+        red[b][1][lane] = h0.y;
+        red[b][2][lane] = h1.x;
+        red[b][3][lane] = h1.y;
+        red[b][4][lane] = rD0;
+        red[b][5][lane] = D2;
+        rD0p = rD0; D2p = D2; h0p = h0; h1p = h1;
       }
-      const double T = t0 / red[b][2][lane];
-      const double B = t1 - red[b][3][lane] * T;
-      red[b][0][lane] = T;   // solution (T_b, B_b) into slots 0, 1
-      red[b][1][lane] = B;
-      yT = T;
+      double2 yT = make_double2(0.0, 0.0);  // T of block b+1
+#pragma unroll 1
+      for (int b = P - 1; b >= 0; --b) {
+        double2 t0 = make_double2(red[b][0][lane], red[b][1][lane]);
+        double2 t1 = make_double2(red[b][2][lane], red[b][3][lane]);
+        if (b < P - 1) {
+          const double ws = red[b][6][lane], we = red[b][7][lane];
+          t0.x -= ws * yT.x; t0.y -= ws * yT.y;
+          t1.x -= we * yT.x; t1.y -= we * yT.y;
+        }
+        const double rD0 = red[b][4][lane], D2 = red[b][5][lane];
+        const double2 T = make_double2(t0.x * rD0, t0.y * rD0);
+        red[b][0][lane] = T.x;  // solution (T_b, B_b) into slots 0..3
+        red[b][1][lane] = T.y;
+        red[b][2][lane] = t1.x - D2 * T.x;
+        red[b][3][lane] = t1.y - D2 * T.y;
+        yT = T;
+      }
     }
+    __syncthreads();
   }
-  __syncthreads();
   if (!valid) return;
-  const double xl = c > 0 ? red[c - 1][1][lane] : 0.0;      // x at row s-1
-  const double xr = c < P - 1 ? red[c + 1][0][lane] : 0.0;  // x at row e+1
+  double2 xl = make_double2(0.0, 0.0), xr = make_double2(0.0, 0.0);
+  if constexpr (P > 1) {
+    if (c > 0) xl = make_double2(red[c - 1][2][lane], red[c - 1][3][lane]);      // x at row s-1
+    if (c < P - 1) xr = make_double2(red[c + 1][0][lane], red[c + 1][1][lane]);  // x at row e+1
+  }
 #pragma unroll
   for (int i = 0; i < M; ++i) {
     const int z = s + i;
-    if (z < n3) xo[z * plane2 + c2] = active ? gv[i] - xl * vv[i] - xr * wv[i] : 0.0;
+    if (z < n3) {
+      double2 x = make_double2(0.0, 0.0);
+      if (active) {
+        x.x = gv[i].x - xl.x * vv[i] - xr.x * wv[i];
+        x.y = gv[i].y - xl.y * vv[i] - xr.y * wv[i];
+      }
+      out[z * plane + ci] = x;
+    }
   }
 }
 
@@ -551,24 +575,55 @@ hipError_t launch_c2r(const Dev& d, hipStream_t st, const double2* k, double* r,
 }
 
 template <int M, int P, int LPC>
-static hipError_t solve_p(const Dev& d, hipStream_t st, int f_lo, int nf, const double2* rhs_psi) {
+static hipError_t solve_c(const Dev& d, hipStream_t st, int f_lo, int nf, const double2* rhs_psi) {
   const int ncols = d.g.KX * d.g.nbox;
-  constexpr int CPB = LPC / 2;
-  dim3 grid((ncols + CPB - 1) / CPB, nf);
-  hipLaunchKernelGGL((k_solve_p<M, P, LPC>), grid, dim3(LPC * P), 0, st, d, f_lo, rhs_psi);
+  dim3 grid((ncols + LPC - 1) / LPC, nf);
+  hipLaunchKernelGGL((k_solve_c<M, P, LPC>), grid, dim3(LPC * P), 0, st, d, f_lo, rhs_psi);
   return hipGetLastError();
+}
+
+template <int M, int LPC>
+static hipError_t solve_m(const Dev& d, hipStream_t st, int f_lo, int nf, const double2* rhs_psi, int P) {
+  switch (P) {
+    case 1: return solve_c<M, 1, LPC>(d, st, f_lo, nf, rhs_psi);
+    case 2: return solve_c<M, 2, LPC>(d, st, f_lo, nf, rhs_psi);
+    case 4: return solve_c<M, 4, LPC>(d, st, f_lo, nf, rhs_psi);
+    case 8: return solve_c<M, 8, LPC>(d, st, f_lo, nf, rhs_psi);
+    case 16: return solve_c<M, 16, LPC>(d, st, f_lo, nf, rhs_psi);
+    default: return solve_c<M, 32, LPC>(d, st, f_lo, nf, rhs_psi);
+  }
+}
+
+// Shape of the partitioned solve for a solve variant v: M levels per chunk
+// and LPC columns per block; P = the smallest power of two with M * P >= n3.
+void solve_shape(int n3, int v, int* M, int* P, int* LPC) {
+  static const int tm[4] = {8, 8, 16, 16}, tl[4] = {16, 32, 16, 32};
+  v = (v < 0 || v > 3) ? 0 : v;
+  int m = tm[v];
+  int p = 1;
+  while (m * p < n3) p *= 2;
+  // at most 512 threads and 32 KiB of LDS per block when the levels allow
+  while (m < 16 && (p > 32 || p * tl[v] > 512)) {
+    m *= 2;
+    p = 1;
+    while (m * p < n3) p *= 2;
+  }
+  *M = m;
+  *P = p;
+  *LPC = p * tl[v] > 512 ? 16 : tl[v];
 }
 
 hipError_t launch_solve(const Dev& d, hipStream_t st, int f_lo, int f_hi, const double2* rhs_psi) {
   if (f_hi < f_lo) return hipSuccess;
   const int nf = f_hi - f_lo + 1;
-  const int n3 = d.g.n3;
-  if (n3 <= 8) return solve_p<8, 1, 64>(d, st, f_lo, nf, rhs_psi);
-  if (n3 <= 16) return solve_p<8, 2, 64>(d, st, f_lo, nf, rhs_psi);
-  if (n3 <= 32) return solve_p<8, 4, 64>(d, st, f_lo, nf, rhs_psi);
-  if (n3 <= 64) return solve_p<8, 8, 64>(d, st, f_lo, nf, rhs_psi);
-  if (n3 <= 128) return solve_p<8, 16, 64>(d, st, f_lo, nf, rhs_psi);
-  if (n3 <= 256) return solve_p<16, 16, 32>(d, st, f_lo, nf, rhs_psi);
+  int M, P, LPC;
+  solve_shape(d.g.n3, d.tune.solve, &M, &P, &LPC);
+  if (P <= 32) {
+    if (M == 8) return LPC == 16 ? solve_m<8, 16>(d, st, f_lo, nf, rhs_psi, P)
+                                 : solve_m<8, 32>(d, st, f_lo, nf, rhs_psi, P);
+    return LPC == 16 ? solve_m<16, 16>(d, st, f_lo, nf, rhs_psi, P) : solve_m<16, 32>(d, st, f_lo, nf, rhs_psi, P);
+  }
+  // more than 512 levels: one sequential Thomas sweep per column
   const int ncols = d.g.KX * d.g.nbox;
   dim3 grid((2 * ncols + NT - 1) / NT, nf);
   hipLaunchKernelGGL(k_solve, grid, dim3(NT), 0, st, d, f_lo, rhs_psi);

@@ -22,6 +22,7 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libqgybj.so")
 
 QG_OK = 0
 ERRORS = {1: "QG_ERR_ARG", 2: "QG_ERR_HIP", 3: "QG_ERR_NOMEM", 4: "QG_ERR_NONFINITE", 5: "QG_ERR_STATE"}
+NVARIANTS = 7  # QG_NVARIANTS
 KERNEL_NAMES = ["col_inv", "row_nl", "col_upd", "row_qw", "col_qw", "solve", "solve_a", "aux"]
 
 # exported symbols declared in include/qgybj.h
@@ -213,14 +214,17 @@ class Member:
         self._chk(self.lib.qg_fft_c2r(self._h, _dp(k), _dp(r), ldx, nz), "qg_fft_c2r")
         return r[..., : self.n1]
 
-    def set_variants(self, e16):
-        a = np.ascontiguousarray(e16, dtype=np.int32)
+    def set_variants(self, v):
+        """v[0..4]: transform-kernel variants, v[5]: solve shape, v[6]: A
+        inversions on the member's stream (1) or a side stream (0)."""
+        a = np.ascontiguousarray(v, dtype=np.int32)
         self._chk(self.lib.qg_set_variants(self._h, a.ctypes.data_as(C.POINTER(C.c_int32)), len(a)),
                   "qg_set_variants")
 
     def get_variants(self):
-        a = np.zeros(5, dtype=np.int32)
-        self._chk(self.lib.qg_get_variants(self._h, a.ctypes.data_as(C.POINTER(C.c_int32)), 5), "qg_get_variants")
+        a = np.zeros(NVARIANTS, dtype=np.int32)
+        self._chk(self.lib.qg_get_variants(self._h, a.ctypes.data_as(C.POINTER(C.c_int32)), NVARIANTS),
+                  "qg_get_variants")
         return a
 
     def sync(self):
