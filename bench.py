@@ -1,6 +1,6 @@
 """Benchmark: member-steps/s of the QG-YBJ+ time step at 256^2 x 128, FP64.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--variants v0,..,v6]
 
 Workload (the plain drop-in at production size): n1 = n2 = 256, n3 = 128,
 FP64, exponential N^2(z) with surface/bottom ratio 100, seeded random-phase
@@ -10,10 +10,15 @@ q^w feedback, psi and A inversions) on the resident state.  The Euler first
 step and W leapfrog warm-up steps run untimed; then exactly K leapfrog steps
 are timed between two stream synchronisations.  Prints one JSON line.
 
-The roofline entry is for the kernel with the largest share of the timed
-region, from HIP events recorded on the member's stream around every launch;
-its algorithmic bytes per launch are the box-restricted reads and writes the
-kernel must do (see KERNEL_BYTES).  The CPU baseline is the FP64 NumPy
+Before the timed region, --profile-steps untimed steps run with every
+kernel timed by HIP events; they give the kernel table and pick the dominant
+kernel (largest total time).  In the timed region only the dominant kernel
+carries HIP events, recorded on the stream it is launched on; the roofline
+entry divides its algorithmic bytes per launch (the box-restricted reads and
+writes the kernel must do, kernel_bytes) by its average duration there.
+"traffic" is the HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE
+counters, scaled by a 1 GiB copy's readings (profiles/*_traffic.json;
+--calib-copy runs that copy).  The CPU baseline is the FP64 NumPy
 restatement (oracle) timed on the host on a bounded sample of the same
 workload.
 """
@@ -33,7 +38,7 @@ HBM_PEAK_GBS = 8000.0
 N1, N3, K0, SEED = 256, 128, 5, 1
 
 
-def kernel_bytes(n1: int, n3: int):
+def kernel_bytes(n1: int, n3: int, a_main: int = 0):
     """Algorithmic HBM bytes per launch of each kernel of a leapfrog step.
 
     G = one half-transformed intermediate restricted to the kept columns
@@ -45,12 +50,12 @@ def kernel_bytes(n1: int, n3: int):
     G = KX * n1 * n3 * 16
     C = KX * nbox * n3 * 16
     return {
-        "col_inv": G + C,          # psi box in, one intermediate out
+        "col_inv": 3 * G + C,      # psi box in; u, v, zeta intermediates out
         "row_nl": 12 * G,          # 6 intermediates in, 6 out
         "col_upd": 11 * G + 14 * C,  # 6 in + 5 out; q,qo,BR,BRo,BI,BIo,AR,AI in, 6 out
         "row_qw": 6 * G,           # 4 in, 2 out
         "col_qw": 2 * G + 2 * C,   # 2 in, q in, q* out
-        "solve": 2 * C,            # q* in, psi out
+        "solve": 6 * C if a_main else 2 * C,  # q* in, psi out (+ BR, BI in, AR, AI out)
         "solve_a": 4 * C,          # BR, BI in; AR, AI out (side stream)
     }, {"G": G, "C": C, "KX": KX, "nbox": nbox}
 
@@ -76,14 +81,36 @@ def cpu_baseline(P, psi, BR, BI, nsteps: int = 2):
                       f"FFTs on {workers} threads, elementwise work single-threaded)"}
 
 
+def calib_copy():
+    """Three copies of 1 GiB between device buffers: the reference kernel whose
+    FETCH_SIZE / WRITE_SIZE readings scale the counters of the step kernels."""
+    import torch
+    n = 1 << 27
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "launches": 3}))
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--profile-steps", type=int, default=20,
+                    help="untimed steps with every kernel timed (kernel table, dominant kernel)")
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--variants", type=str, default="",
+                    help="comma-separated kernel variants (qg_set_variants), default: the library's")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="only run three 1 GiB device copies (HBM counter calibration under rocprofv3 --pmc)")
     a = ap.parse_args(argv)
+    if a.calib_copy:
+        return calib_copy()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs one member per GPU on a single MI355X; "
               "multi-GPU runs (--gpus > 1) are not supported", file=sys.stderr)
@@ -98,12 +125,27 @@ def main(argv=None):
     psi = ic.eddy_psi(N1, N3, K0, SEED, Bu=P.Bu)
     BR, BI = ic.wave_b(N1, N3)
     m = qgybj.Member(P)
+    if a.variants:
+        m.set_variants([int(x) for x in a.variants.split(",")])
+    variants = [int(x) for x in m.get_variants()]
     m.set_state_physical(psi, BR, BI)
     m.step(1)                 # Euler first step (untimed)
     m.step(a.warmup)          # leapfrog warm-up
     m.sync()
+    # per-kernel profile pass (every kernel timed with HIP events; untimed
+    # region): the kernel table and the choice of the dominant kernel
+    kb, geo = kernel_bytes(N1, N3, variants[6])
     m.kernel_times()          # reset
     m.timing(True)
+    m.step(a.profile_steps)
+    m.timing(False)
+    kt_all = m.kernel_times()
+    dom = max((k for k in kb), key=lambda k: kt_all[k][0])
+
+    # timed region: only the dominant kernel carries HIP events (two event
+    # records per step on its stream)
+    m.sync()
+    m.timing(dom)
     t0 = time.perf_counter()
     m.step(a.steps)           # ends with a stream synchronisation
     t1 = time.perf_counter()
@@ -117,8 +159,6 @@ def main(argv=None):
         print("bench.py: non-finite diagnostics", file=sys.stderr)
         return 1
 
-    kb, geo = kernel_bytes(N1, N3)
-    dom = max((k for k in kb), key=lambda k: kt[k][0])
     tot_ms, cnt = kt[dom]
     avg_s = tot_ms * 1e-3 / max(cnt, 1)
     achieved = kb[dom] / avg_s / 1e9
@@ -131,8 +171,8 @@ def main(argv=None):
             roofline["traffic"] = json.load(open(traffic_file)).get("bytes_per_launch")
         except Exception:
             pass
-    kernels = {k: {"ms_total": round(v[0], 3), "launches": v[1]} for k, v in kt.items() if v[1]}
-    step_alg = sum(kb[k] * kt[k][1] for k in kb) / a.steps
+    kernels = {k: {"avg_us": round(1e3 * v[0] / v[1], 2), "launches": v[1]} for k, v in kt_all.items() if v[1]}
+    step_alg = sum(kb[k] * kt_all[k][1] for k in kb) / a.profile_steps
     m.close()
 
     cpu = None
@@ -154,7 +194,7 @@ def main(argv=None):
         "data": "synthetic",
         "config": {"workload": "plain drop-in step, 256^2x128 FP64, exponential N^2 (ratio 100), "
                                "eddies peaked at |k|=5 (seed 1), surface-trapped waves, feedback on",
-                   "n1": N1, "n2": N1, "n3": N3, "members": 1},
+                   "n1": N1, "n2": N1, "n3": N3, "members": 1, "variants": variants},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": kernels,

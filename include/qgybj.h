@@ -132,12 +132,16 @@ int qg_device_pitch(const qg_ctx* ctx, int32_t* pitch);
 int qg_stream(qg_ctx* ctx, void** stream);
 int qg_sync(qg_ctx* ctx);
 
-/* Per-kernel timing with HIP events on the stream each kernel runs on:
- * enable, then read accumulated milliseconds and launch counts per kernel id
- * (0 col_inv, 1 row_nl, 2 col_upd, 3 row_qw, 4 col_qw, 5 solve (psi, and A
- * when the A inversions run on the member's stream), 6 solve_a (A alone),
- * 7 aux).  Reading resets the accumulators. */
-int qg_timing(qg_ctx* ctx, int32_t enable);
+/* Per-kernel timing with HIP events on the stream each kernel runs on.
+ * mask: bit k times kernel id k (0 col_inv, 1 row_nl, 2 col_upd, 3 row_qw,
+ * 4 col_qw, 5 solve (psi, and A when the A inversions run on the member's
+ * stream), 6 solve_a (A alone), 7 aux); QG_TIME_ALL times every kernel, 0
+ * stops timing.  qg_kernel_times reads accumulated milliseconds and launch
+ * counts per kernel id and resets the accumulators.  Each timed launch adds
+ * two event records to its stream, so time only what is needed where the
+ * throughput itself is being measured. */
+enum { QG_TIME_ALL = 0xFF };
+int qg_timing(qg_ctx* ctx, int32_t mask);
 int qg_kernel_times(qg_ctx* ctx, double* ms, int32_t* counts, int32_t n);
 const char* qg_kernel_name(int32_t id);
 

@@ -64,7 +64,7 @@ struct qg_ctx {
   int clear_pending = 0;  // steps that still zero the modes outside the box
   std::string err;
   std::vector<void*> allocs;
-  bool timing = false;
+  uint32_t timing = 0;  // bit k: time kernel id k
   struct Ev {
     int id;
     hipEvent_t a, b;
@@ -141,7 +141,7 @@ struct Timed {
   hipStream_t st;
   hipEvent_t a = nullptr;
   Timed(qg_ctx* cc, int i, hipStream_t s) : c(cc), id(i), st(s) {
-    if (c->timing && hipEventCreate(&a) == hipSuccess) hipEventRecord(a, st);
+    if (((c->timing >> id) & 1u) && hipEventCreate(&a) == hipSuccess) hipEventRecord(a, st);
   }
   void done() {
     if (a) {
@@ -627,9 +627,9 @@ int qg_sync(qg_ctx* c) {
   return QG_OK;
 }
 
-int qg_timing(qg_ctx* c, int32_t enable) {
+int qg_timing(qg_ctx* c, int32_t mask) {
   if (!c) return QG_ERR_ARG;
-  c->timing = enable != 0;
+  c->timing = (uint32_t)mask & ((1u << qg::K_NKERNELS) - 1u);
   return QG_OK;
 }
 

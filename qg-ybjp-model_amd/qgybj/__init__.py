@@ -231,7 +231,16 @@ class Member:
         self._chk(self.lib.qg_sync(self._h), "qg_sync")
 
     def timing(self, on=True):
-        self._chk(self.lib.qg_timing(self._h, 1 if on else 0), "qg_timing")
+        """on: True (every kernel), False, a kernel name or an id bit mask."""
+        if on is True:
+            mask = 0xFF
+        elif on is False or on is None:
+            mask = 0
+        elif isinstance(on, str):
+            mask = 1 << KERNEL_NAMES.index(on)
+        else:
+            mask = int(on)
+        self._chk(self.lib.qg_timing(self._h, mask), "qg_timing")
 
     def kernel_times(self):
         n = len(KERNEL_NAMES)
