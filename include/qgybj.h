@@ -17,9 +17,12 @@
  *    passes ldx = n1+2, main_waqg.f90:15-110).
  *  - Every call returns QG_OK (0) or an error code; the library never exits.
  *    qg_last_error() gives the message of the last failure on a context.
- *  - One context = one member on one HIP stream of the current device.  Calls
- *    on one context are serialised by the caller; distinct contexts are
- *    independent.
+ *  - One context = one member, or a batch of `members` members with the same
+ *    grid and parameters (qg_create_batch), on one HIP stream of the current
+ *    device.  For a batch every array argument carries a trailing member
+ *    dimension: X(iktx, ikty, n3, members), X(ldx, n2, nz) with nz counting
+ *    the levels of all members (member-major).  Calls on one context are
+ *    serialised by the caller; distinct contexts are independent.
  */
 #ifndef QGYBJ_H
 #define QGYBJ_H
@@ -78,6 +81,13 @@ enum { QG_PSI = 0, QG_Q, QG_QO, QG_BR, QG_BI, QG_BRO, QG_BIO, QG_AR, QG_AI, QG_N
  * (init.f90:177-264): allocates the member's device state, twiddles, masks,
  * integrating-factor exponentials and vertical operator. */
 int qg_create(const qg_params* p, qg_ctx** out);
+/* A batch of `members` independent members with the same qg_params (their
+ * states differ): every launch of the step covers all members, which fills
+ * the GPU when one member alone cannot.  qg_create(p, out) is
+ * qg_create_batch(p, 1, out).  No counterpart in the reference (one member
+ * per run). */
+int qg_create_batch(const qg_params* p, int32_t members, qg_ctx** out);
+int qg_members(const qg_ctx* ctx, int32_t* members);
 int qg_destroy(qg_ctx* ctx);
 const char* qg_last_error(const qg_ctx* ctx);
 
@@ -107,17 +117,18 @@ int qg_get_state(qg_ctx* ctx, double* psik, double* qk, double* qok, double* BRk
 int qg_step(qg_ctx* ctx, int32_t nsteps);
 int qg_iter(const qg_ctx* ctx, int32_t* iter);
 
-/* FP64 diagnostics of the current state, nondimensional:
+/* FP64 diagnostics of the current state (6 values per member, member-major),
+ * nondimensional:
  * out[0] KE and out[1] PE (diag_zentrum ks / ps_quad, diagnostics.f90:127-158),
  * out[2] WKE, out[3] WPE, out[4] WCE (wave_energy k_p, p_p, c_p,
  * diagnostics.f90:683-712, without the Uw_scale^2 factor),
  * out[5] enstrophy 0.5<zeta^2> = sum kh2^2 |psi|^2; all averaged over z. */
-int qg_diag(qg_ctx* ctx, double out[6]);
+int qg_diag(qg_ctx* ctx, double* out);
 
 /* init_q (init.f90:591-652) on host arrays: q = T(kh2) psi / dz^2 at L = 1. */
 int qg_init_q(qg_ctx* ctx, const double* psik, double* qk);
 
-/* fft_r2c / fft_c2r (fft.f90:55-83) on host arrays of nz <= n3 levels.
+/* fft_r2c / fft_c2r (fft.f90:55-83) on host arrays of nz <= n3 * members levels.
  * r2c is unmasked and scaled by 1/(n1 n2); c2r applies realit and is
  * unnormalised.  Unlike the reference, the c2r input is not modified. */
 int qg_fft_r2c(qg_ctx* ctx, const double* r, int32_t ldx, double* k, int32_t nz);
@@ -159,6 +170,32 @@ const char* qg_kernel_name(int32_t id);
 enum { QG_NVARIANTS = 7 };
 int qg_set_variants(qg_ctx* ctx, const int32_t* v, int32_t n);
 int qg_get_variants(const qg_ctx* ctx, int32_t* v, int32_t n);
+
+/* Ensembles: contexts (members or batches, any mix of sizes and parameters,
+ * one device) advanced together.  qg_ensemble_step(e, n) advances every
+ * context n steps: after the first steps (Euler, full-width transforms,
+ * clearing) it replays one captured HIP graph per k steps (k = 8, or
+ * qg_ensemble_set_graph_steps), in which every context's kernels run on its
+ * own streams, forked from and joined to one origin stream, so the members'
+ * kernels overlap on the device.  Results are those of qg_step on each
+ * context.  A context may belong to one ensemble at a time and must not be
+ * stepped by other calls while an ensemble step runs.  Changing a context's
+ * variants makes the ensemble capture its graphs again.  No counterpart in
+ * the reference. */
+typedef struct qg_ens qg_ens;
+int qg_ensemble_create(qg_ctx* const* members, int32_t n, qg_ens** out);
+int qg_ensemble_destroy(qg_ens* e);
+const char* qg_ensemble_error(const qg_ens* e);
+int qg_ensemble_set_graph_steps(qg_ens* e, int32_t k);
+int qg_ensemble_step(qg_ens* e, int32_t nsteps);
+
+/* Tuner: coordinate search over the QG_NVARIANTS kernel variants of one
+ * context, timing `steps` whole steps per trial with HIP events from the
+ * context's current state (saved before and restored after, so the state is
+ * unchanged).  Leaves the fastest variants set; best (QG_NVARIANTS values) and
+ * best_ms (per step) may be NULL.  Needs device memory for one copy of the
+ * state.  No counterpart in the reference. */
+int qg_tune(qg_ctx* ctx, int32_t steps, int32_t* best, double* best_ms);
 
 /* Host-only helpers (no GPU needed): the dealiasing mask L(iktx, ikty) as the
  * library evaluates it, and the box geometry {kmax, KX, nlo, nhi}. */

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "qg_common.hpp"
+#include "qg_ctx.hpp"
 #include "qg_fft.hpp"
 
 using qg::Dev;
@@ -18,11 +19,13 @@ using qg::Geo;
 
 namespace qg {
 
-Geo make_geo(int n1, int n2, int n3) {
+Geo make_geo(int n1, int n2, int n3, int nm) {
   Geo g{};
   g.n1 = n1;
   g.n2 = n2;
   g.n3 = n3;
+  g.nm = nm;
+  g.nz = nm * n3;
   g.iktx = n1 / 2 + 1;
   g.ikty = n2;
   g.sp = (g.iktx + 7) & ~7;
@@ -51,30 +54,8 @@ Geo make_geo(int n1, int n2, int n3) {
 
 }  // namespace qg
 
-struct qg_ctx {
-  Dev d{};
-  hipStream_t stream = nullptr;
-  // side stream for the B -> A inversions, which only the next step's update
-  // reads: evB marks B^{n+1} written, evA marks A^{n+1} written
-  hipStream_t stream2 = nullptr;
-  hipEvent_t evB = nullptr, evA = nullptr;
-  int device = 0;
-  int iter = 0;
-  int p_valid = 0;        // I0, I2, I4 hold the inverse-y transforms of q, BR, BI
-  int clear_pending = 0;  // steps that still zero the modes outside the box
-  std::string err;
-  std::vector<void*> allocs;
-  uint32_t timing = 0;  // bit k: time kernel id k
-  struct Ev {
-    int id;
-    hipEvent_t a, b;
-  };
-  std::vector<Ev> events;
-  double tms[qg::K_NKERNELS] = {};
-  int tcnt[qg::K_NKERNELS] = {};
-};
 
-namespace {
+namespace qgi {
 
 int fail(qg_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
@@ -84,11 +65,6 @@ int fail_hip(qg_ctx* c, hipError_t e, const char* what) {
   return fail(c, QG_ERR_HIP, std::string("HIP error '") + hipGetErrorString(e) + "' in " + what);
 }
 
-#define HIPCHK(c, call)                                   \
-  do {                                                    \
-    hipError_t _e = (call);                               \
-    if (_e != hipSuccess) return fail_hip((c), _e, #call); \
-  } while (0)
 
 template <class T>
 int dalloc(qg_ctx* c, T** p, size_t n) {
@@ -100,7 +76,7 @@ int dalloc(qg_ctx* c, T** p, size_t n) {
   return QG_OK;
 }
 
-size_t field_elems(const Geo& g) { return (size_t)g.n3 * g.ikty * g.sp; }
+size_t field_elems(const Geo& g) { return (size_t)g.nz * g.ikty * g.sp; }
 
 // host (iktx, ikty, nz) complex <-> device rows of pitch sp
 hipError_t h2d_field(const Geo& g, double2* dst, const double* src, int nz, hipStream_t st) {
@@ -141,7 +117,7 @@ struct Timed {
   hipStream_t st;
   hipEvent_t a = nullptr;
   Timed(qg_ctx* cc, int i, hipStream_t s) : c(cc), id(i), st(s) {
-    if (((c->timing >> id) & 1u) && hipEventCreate(&a) == hipSuccess) hipEventRecord(a, st);
+    if (!c->capturing && ((c->timing >> id) & 1u) && hipEventCreate(&a) == hipSuccess) hipEventRecord(a, st);
   }
   void done() {
     if (a) {
@@ -188,17 +164,17 @@ int step_once(qg_ctx* c) {
   const int box = full ? 0 : 1;
   // u, v, zeta ingredients from psi (compute_velo derivatives.f90:1302-1311,
   // refraction_waqg :831-845)
-  KRUN(c, qg::K_COL_INV, qg::launch_col_inv(d, st, d.psi, d.I[1], d.I[3], d.I[5], W, box, 1, g.n3));
+  KRUN(c, qg::K_COL_INV, qg::launch_col_inv(d, st, d.psi, d.I[1], d.I[3], d.I[5], W, box, 1, g.nz));
   if (full) {
     // unmasked fields as the state holds them (the initial B is un-dealiased)
-    KRUN(c, qg::K_COL_INV, qg::launch_col_inv(d, st, d.q, d.I[0], nullptr, nullptr, W, 0, 0, g.n3));
-    KRUN(c, qg::K_COL_INV, qg::launch_col_inv(d, st, d.BR, d.I[2], nullptr, nullptr, W, 0, 0, g.n3));
-    KRUN(c, qg::K_COL_INV, qg::launch_col_inv(d, st, d.BI, d.I[4], nullptr, nullptr, W, 0, 0, g.n3));
+    KRUN(c, qg::K_COL_INV, qg::launch_col_inv(d, st, d.q, d.I[0], nullptr, nullptr, W, 0, 0, g.nz));
+    KRUN(c, qg::K_COL_INV, qg::launch_col_inv(d, st, d.BR, d.I[2], nullptr, nullptr, W, 0, 0, g.nz));
+    KRUN(c, qg::K_COL_INV, qg::launch_col_inv(d, st, d.BI, d.I[4], nullptr, nullptr, W, 0, 0, g.nz));
   }
   KRUN(c, qg::K_ROW_NL, qg::launch_row_nl(d, st, W));
   // the update reads A^n and overwrites B in place: wait for the previous
   // step's A inversions on the side stream
-  HIPCHK(c, hipStreamWaitEvent(st, c->evA, 0));
+  if (c->evA_live) HIPCHK(c, hipStreamWaitEvent(st, c->evA, 0));
   KRUN(c, qg::K_COL_UPD, qg::launch_col_upd(d, st, euler ? 1 : 0, has_dq));
   c->p_valid = 1;
   const bool solve_a = !(flags & qg::F_PASSIVE);
@@ -210,6 +186,7 @@ int step_once(qg_ctx* c) {
     HIPCHK(c, hipStreamWaitEvent(c->stream2, c->evB, 0));
     KRUN_ON(c, c->stream2, qg::K_SOLVE_A, qg::launch_solve(d, c->stream2, 1, 2, nullptr));
     HIPCHK(c, hipEventRecord(c->evA, c->stream2));
+    c->evA_live = 1;
   }
   // the inversions left for the member's stream: psi (field 0) unless the
   // flow is fixed, then AR, AI (fields 1, 2) unless they ran on the side
@@ -227,7 +204,7 @@ int step_once(qg_ctx* c) {
     KRUN(c, qg::K_SOLVE_A, qg::launch_solve(d, st, f_lo, f_hi, nullptr));
   }
   if (c->clear_pending > 0) {
-    HIPCHK(c, hipStreamWaitEvent(st, c->evA, 0));
+    if (c->evA_live) HIPCHK(c, hipStreamWaitEvent(st, c->evA, 0));
     // modes outside the dealiasing box are zero in the reference after the
     // masked update / filter / inversions; the hot loop never touches them
     KRUN(c, qg::K_AUX, qg::launch_clear_outside(d, st, d.q, 0));
@@ -255,7 +232,9 @@ int check_ctx(qg_ctx* c) {
   return QG_OK;
 }
 
-}  // namespace
+}  // namespace qgi
+
+using namespace qgi;
 
 extern "C" {
 
@@ -301,10 +280,19 @@ int qg_destroy(qg_ctx* c) {
   return QG_OK;
 }
 
-int qg_create(const qg_params* p, qg_ctx** out) {
+int qg_create(const qg_params* p, qg_ctx** out) { return qg_create_batch(p, 1, out); }
+
+int qg_members(const qg_ctx* c, int32_t* members) {
+  if (!c || !members) return QG_ERR_ARG;
+  *members = c->d.g.nm;
+  return QG_OK;
+}
+
+int qg_create_batch(const qg_params* p, int32_t members, qg_ctx** out) {
   if (!p || !out) return QG_ERR_ARG;
   *out = nullptr;
   if (p->n1 != p->n2 || !qg::supported_n(p->n1) || p->n3 < 2) return QG_ERR_ARG;
+  if (members < 1 || (int64_t)members * p->n3 > (1 << 20)) return QG_ERR_ARG;
   if (!p->ifac_q || !p->ifac_w || !p->N2_u || !p->N2_s) return QG_ERR_ARG;
   if (!(p->dt > 0.0) || !std::isfinite(p->gamma) || !(p->Ro > 0.0) || !(p->Bu > 0.0)) return QG_ERR_ARG;
   qg_ctx* c = new qg_ctx();
@@ -324,7 +312,7 @@ int qg_create(const qg_params* p, qg_ctx** out) {
     return QG_ERR_HIP;
   }
   Dev& d = c->d;
-  d.g = qg::make_geo(p->n1, p->n2, p->n3);
+  d.g = qg::make_geo(p->n1, p->n2, p->n3, members);
   const Geo& g = d.g;
   // measured default at 256^2 x 128: 16-level chunks, 32 columns per block,
   // A inversions on the side stream
@@ -344,15 +332,15 @@ int qg_create(const qg_params* p, qg_ctx** out) {
   for (auto f : fields)
     if ((rc = dalloc(c, f, S)) != QG_OK) { qg_destroy(c); return rc; }
   for (int i = 0; i < 6; ++i)
-    if ((rc = dalloc(c, &d.I[i], (size_t)g.n3 * g.n2 * g.sp)) != QG_OK) { qg_destroy(c); return rc; }
+    if ((rc = dalloc(c, &d.I[i], (size_t)g.nz * g.n2 * g.sp)) != QG_OK) { qg_destroy(c); return rc; }
   const size_t ncols = (size_t)g.KX * g.nbox;
-  if ((rc = dalloc(c, &d.scratch, 3 * (size_t)g.n3 * ncols * 2)) != QG_OK) { qg_destroy(c); return rc; }
+  if ((rc = dalloc(c, &d.scratch, 3 * (size_t)g.nz * ncols * 2)) != QG_OK) { qg_destroy(c); return rc; }
   double2* tw = nullptr;
   if ((rc = dalloc(c, &tw, (size_t)qg::TWN)) != QG_OK) { qg_destroy(c); return rc; }
   d.tw = tw;
   double* tab = nullptr;
   if ((rc = dalloc(c, &tab, 4 * plane + 6 * (size_t)g.n3)) != QG_OK) { qg_destroy(c); return rc; }
-  if ((rc = dalloc(c, &d.diag_part, (size_t)qg::diag_blocks(g) * 6)) != QG_OK) { qg_destroy(c); return rc; }
+  if ((rc = dalloc(c, &d.diag_part, (size_t)qg::diag_blocks(g) * g.nm * 6)) != QG_OK) { qg_destroy(c); return rc; }
   if ((rc = dalloc(c, &d.flag, 1)) != QG_OK) { qg_destroy(c); return rc; }
 
   // host tables
@@ -426,7 +414,7 @@ int qg_set_state(qg_ctx* c, const double* psik, const double* qk, const double* 
   double2* dst[] = {d.psi, d.q, d.qo, d.BR, d.BI, d.BRo, d.BIo, d.AR, d.AI};
   for (int i = 0; i < 9; ++i) {
     HIPCHK(c, qg::launch_clear_outside(d, c->stream, dst[i], 1));
-    if (src[i]) HIPCHK(c, h2d_field(d.g, dst[i], src[i], d.g.n3, c->stream));
+    if (src[i]) HIPCHK(c, h2d_field(d.g, dst[i], src[i], d.g.nz, c->stream));
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->iter = iter;
@@ -441,14 +429,14 @@ int qg_set_state_physical(qg_ctx* c, const double* psir, const double* BRr, cons
   Dev& d = c->d;
   const Geo& g = d.g;
   if (!psir || !BRr || !BIr || ldx < g.n1) return fail(c, QG_ERR_ARG, "null input or ldx < n1");
-  const size_t nr = (size_t)ldx * g.n2 * g.n3;
+  const size_t nr = (size_t)ldx * g.n2 * g.nz;
   double* tmp = nullptr;
   HIPCHK(c, hipMalloc(&tmp, nr * sizeof(double)));
   const double* srcs[] = {psir, BRr, BIr};
   double2* dsts[] = {d.psi, d.BR, d.BI};
   for (int i = 0; i < 3; ++i) {
     hipError_t e = hipMemcpyAsync(tmp, srcs[i], nr * sizeof(double), hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess) e = qg::launch_r2c(d, c->stream, tmp, ldx, dsts[i], g.n3);
+    if (e == hipSuccess) e = qg::launch_r2c(d, c->stream, tmp, ldx, dsts[i], g.nz);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
       hipFree(tmp);
@@ -475,14 +463,14 @@ int qg_get_state(qg_ctx* c, double* psik, double* qk, double* qok, double* BRk, 
   double* dst[] = {psik, qk, qok, BRk, BIk, BRok, BIok, ARk, AIk};
   const double2* src[] = {d.psi, d.q, d.qo, d.BR, d.BI, d.BRo, d.BIo, d.AR, d.AI};
   for (int i = 0; i < 9; ++i)
-    if (dst[i]) HIPCHK(c, d2h_field(d.g, dst[i], src[i], d.g.n3, c->stream));
+    if (dst[i]) HIPCHK(c, d2h_field(d.g, dst[i], src[i], d.g.nz, c->stream));
   if (CRk) {
     HIPCHK(c, qg::launch_cz(d, c->stream, d.AR, d.qs));
-    HIPCHK(c, d2h_field(d.g, CRk, d.qs, d.g.n3, c->stream));
+    HIPCHK(c, d2h_field(d.g, CRk, d.qs, d.g.nz, c->stream));
   }
   if (CIk) {
     HIPCHK(c, qg::launch_cz(d, c->stream, d.AI, d.qs));
-    HIPCHK(c, d2h_field(d.g, CIk, d.qs, d.g.n3, c->stream));
+    HIPCHK(c, d2h_field(d.g, CIk, d.qs, d.g.nz, c->stream));
   }
   return QG_OK;
 }
@@ -510,20 +498,22 @@ int qg_iter(const qg_ctx* c, int32_t* iter) {
   return QG_OK;
 }
 
-int qg_diag(qg_ctx* c, double out[6]) {
+int qg_diag(qg_ctx* c, double* out) {
   int rc = check_ctx(c);
   if (rc) return rc;
   if (!out) return fail(c, QG_ERR_ARG, "null output");
   int nb = 0;
   HIPCHK(c, qg::launch_diag(c->d, c->stream, &nb));
-  std::vector<double> part((size_t)nb * 6);
+  const int nm = c->d.g.nm;
+  std::vector<double> part((size_t)nb * nm * 6);
   HIPCHK(c, hipStreamSynchronize(c->stream));
   HIPCHK(c, hipMemcpy(part.data(), c->d.diag_part, part.size() * sizeof(double), hipMemcpyDeviceToHost));
-  for (int j = 0; j < 6; ++j) {
-    double s = 0.0;
-    for (int b = 0; b < nb; ++b) s += part[(size_t)b * 6 + j];
-    out[j] = s / c->d.g.n3;
-  }
+  for (int m = 0; m < nm; ++m)
+    for (int j = 0; j < 6; ++j) {
+      double s = 0.0;
+      for (int b = 0; b < nb; ++b) s += part[((size_t)m * nb + b) * 6 + j];
+      out[m * 6 + j] = s / c->d.g.n3;
+    }
   return QG_OK;
 }
 
@@ -533,9 +523,9 @@ int qg_init_q(qg_ctx* c, const double* psik, double* qk) {
   if (!psik || !qk) return fail(c, QG_ERR_ARG, "null array");
   Dev& d = c->d;
   HIPCHK(c, qg::launch_clear_outside(d, c->stream, d.qs, 1));
-  HIPCHK(c, h2d_field(d.g, d.qs, psik, d.g.n3, c->stream));
+  HIPCHK(c, h2d_field(d.g, d.qs, psik, d.g.nz, c->stream));
   HIPCHK(c, qg::launch_init_q(d, c->stream, d.qs, d.dq));
-  HIPCHK(c, d2h_field(d.g, qk, d.dq, d.g.n3, c->stream));
+  HIPCHK(c, d2h_field(d.g, qk, d.dq, d.g.nz, c->stream));
   return QG_OK;
 }
 
@@ -544,7 +534,7 @@ int qg_fft_r2c(qg_ctx* c, const double* r, int32_t ldx, double* k, int32_t nz) {
   if (rc) return rc;
   Dev& d = c->d;
   const Geo& g = d.g;
-  if (!r || !k || ldx < g.n1 || nz < 1 || nz > g.n3) return fail(c, QG_ERR_ARG, "bad r2c arguments");
+  if (!r || !k || ldx < g.n1 || nz < 1 || nz > g.nz) return fail(c, QG_ERR_ARG, "bad r2c arguments");
   const size_t nr = (size_t)ldx * g.n2 * nz;
   double* tmp = nullptr;
   HIPCHK(c, hipMalloc(&tmp, nr * sizeof(double)));
@@ -562,7 +552,7 @@ int qg_fft_c2r(qg_ctx* c, const double* k, double* r, int32_t ldx, int32_t nz) {
   if (rc) return rc;
   Dev& d = c->d;
   const Geo& g = d.g;
-  if (!r || !k || ldx < g.n1 || nz < 1 || nz > g.n3) return fail(c, QG_ERR_ARG, "bad c2r arguments");
+  if (!r || !k || ldx < g.n1 || nz < 1 || nz > g.nz) return fail(c, QG_ERR_ARG, "bad c2r arguments");
   const size_t nr = (size_t)ldx * g.n2 * nz;
   double* tmp = nullptr;
   HIPCHK(c, hipMalloc(&tmp, nr * sizeof(double)));
@@ -593,6 +583,7 @@ int qg_set_variants(qg_ctx* c, const int32_t* v, int32_t n) {
   }
   // the A inversions change stream: let any in-flight side-stream work end
   if (n > 6) HIPCHK(c, hipStreamWaitEvent(c->stream, c->evA, 0));
+  c->generation++;
   for (int i = 0; i < n && i < QG_NVARIANTS; ++i) {
     if (i < 5) c->d.tune.e16[i] = v[i];
     else if (i == 5) c->d.tune.solve = v[i];

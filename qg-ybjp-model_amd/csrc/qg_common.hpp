@@ -13,6 +13,11 @@ namespace qg {
 // iky >= n2 - nhi, which contains every mode with L = 1 (init.f90:44-80).
 struct Geo {
   int n1, n2, n3;
+  int nm;      // members stacked in one context (same grid and parameters)
+  int nz;      // nm * n3: levels of every device field, member-major
+               // (member m, level k at plane m * n3 + k); the horizontal
+               // kernels treat all nz planes alike, the vertical ones work
+               // per member
   int iktx, ikty;
   int sp;      // device row pitch in complex elements: iktx rounded up to a
                // multiple of 8 so that every 8-column tile is one 128-byte line
@@ -46,7 +51,7 @@ __host__ __device__ inline int box_row(int ii, const Geo& g) {
   return ii < g.nlo ? ii : ii + (g.n2 - g.nlo - g.nhi);
 }
 
-Geo make_geo(int n1, int n2, int n3);
+Geo make_geo(int n1, int n2, int n3, int nm = 1);
 
 // flag bits, same values as include/qgybj.h
 enum : int {

@@ -96,7 +96,7 @@ __global__ __launch_bounds__(NT) void k_col_inv(Geo g, const double2* __restrict
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(NT) void k_solve(Dev d, int f_lo, const double2* __restrict__ rhs_psi) {
   const Geo& g = d.g;
-  const int f = f_lo + blockIdx.y;
+  const int f = f_lo + (int)blockIdx.y / g.nm, mem = (int)blockIdx.y % g.nm;
   const int ncols = g.KX * g.nbox;
   const int gid = blockIdx.x * NT + threadIdx.x;
   const int col = gid >> 1, part = gid & 1;
@@ -106,11 +106,11 @@ __global__ __launch_bounds__(NT) void k_solve(Dev d, int f_lo, const double2* __
   const int kh2 = kx * kx + ky * ky;
   const double2* rhs = f == 0 ? rhs_psi : (f == 1 ? d.BR : d.BI);
   double2* out = f == 0 ? d.psi : (f == 1 ? d.AR : d.AI);
-  const double* rin = reinterpret_cast<const double*>(rhs) + part;
-  double* xo = reinterpret_cast<double*>(out) + part;
   const size_t plane2 = (size_t)2 * g.ikty * g.sp;
-  const size_t c2 = (size_t)2 * ((size_t)iky * g.sp + kx);
   const int n3 = g.n3;
+  const double* rin = reinterpret_cast<const double*>(rhs) + part + (size_t)mem * n3 * plane2;
+  double* xo = reinterpret_cast<double*>(out) + part + (size_t)mem * n3 * plane2;
+  const size_t c2 = (size_t)2 * ((size_t)iky * g.sp + kx);
   if (kh2 == 0 || !lmask(kx, iky, g)) {
     for (int z = 0; z < n3; ++z) xo[z * plane2 + c2] = 0.0;
     return;
@@ -118,7 +118,7 @@ __global__ __launch_bounds__(NT) void k_solve(Dev d, int f_lo, const double2* __
   const double dz = d.dz;
   const double kt = f == 0 ? ((double)kh2 * dz) * dz : (((double)kh2 * dz) * dz) / 4.0;
   const double rs = f == 0 ? dz * dz : (dz * dz) * d.Bu;
-  double* dsc = d.scratch + ((size_t)f * n3 * ncols) * 2 + (size_t)col * 2 + part;
+  double* dsc = d.scratch + ((size_t)(f * g.nm + mem) * n3 * ncols) * 2 + (size_t)col * 2 + part;
   const size_t dst = (size_t)ncols * 2;
   double dp = -(d.dsum[0] + kt);
   double bp = rs * rin[c2];
@@ -157,7 +157,7 @@ template <int M, int P, int LPC>
 __global__ __launch_bounds__(LPC * P) void k_solve_c(Dev d, int f_lo, const double2* __restrict__ rhs_psi) {
   const Geo& g = d.g;
   const int lane = threadIdx.x % LPC, c = threadIdx.x / LPC;
-  const int f = f_lo + blockIdx.y;
+  const int f = f_lo + (int)blockIdx.y / g.nm, mem = (int)blockIdx.y % g.nm;
   const int ncols = g.KX * g.nbox;
   const int col = blockIdx.x * LPC + lane;
   const bool valid = col < ncols;
@@ -169,11 +169,12 @@ __global__ __launch_bounds__(LPC * P) void k_solve_c(Dev d, int f_lo, const doub
     kh2 = kx * kx + ky * ky;
   }
   const bool active = valid && kh2 != 0 && lmask(kx, iky, g);
-  const double2* __restrict__ rhs = f == 0 ? rhs_psi : (f == 1 ? d.BR : d.BI);
-  double2* __restrict__ out = f == 0 ? d.psi : (f == 1 ? d.AR : d.AI);
   const size_t plane = (size_t)g.ikty * g.sp;
-  const size_t ci = (size_t)iky * g.sp + kx;
   const int n3 = g.n3;
+  const size_t moff = (size_t)mem * n3 * plane;
+  const double2* __restrict__ rhs = (f == 0 ? rhs_psi : (f == 1 ? d.BR : d.BI)) + moff;
+  double2* __restrict__ out = (f == 0 ? d.psi : (f == 1 ? d.AR : d.AI)) + moff;
+  const size_t ci = (size_t)iky * g.sp + kx;
   const double dz = d.dz;
   // inactive columns get a regular (kh2 = 1) system so that no lane divides
   // by zero; their output is forced to 0
@@ -318,10 +319,10 @@ __global__ __launch_bounds__(NT) void k_dq(Dev d) {
   const Geo& g = d.g;
   const int per = g.KX * g.nbox;
   const int e = blockIdx.x * NT + threadIdx.x;
-  if (e >= per * g.n3) return;
-  const int z = e / per, r = e % per, kx = r % g.KX, iky = box_row(r / g.KX, g);
+  if (e >= per * g.nz) return;
+  const int zt = e / per, z = zt % g.n3, r = e % per, kx = r % g.KX, iky = box_row(r / g.KX, g);
   const size_t plane = (size_t)g.ikty * g.sp;
-  const size_t idx = z * plane + (size_t)iky * g.sp + kx;
+  const size_t idx = zt * plane + (size_t)iky * g.sp + kx;
   double2 v = zc();
   if (lmask(kx, iky, g)) {
     const double dz2 = d.dz * d.dz;
@@ -344,7 +345,7 @@ __global__ __launch_bounds__(NT) void k_dq(Dev d) {
 __global__ __launch_bounds__(NT) void k_clear(Geo g, double2* f, int all) {
   const size_t plane = (size_t)g.ikty * g.sp;
   const size_t e = (size_t)blockIdx.x * NT + threadIdx.x;
-  if (e >= plane * g.n3) return;
+  if (e >= plane * g.nz) return;
   const size_t r = e % plane;
   const int iky = (int)(r / g.sp), kx = (int)(r % g.sp);
   if (all || kx >= g.KX || !in_box_row(iky, g)) f[e] = zc();
@@ -355,8 +356,8 @@ __global__ __launch_bounds__(NT) void k_init_q(Dev d, const double2* __restrict_
   const Geo& g = d.g;
   const size_t plane = (size_t)g.ikty * g.sp;
   const size_t e = (size_t)blockIdx.x * NT + threadIdx.x;
-  if (e >= plane * g.n3) return;
-  const int z = (int)(e / plane);
+  if (e >= plane * g.nz) return;
+  const int z = (int)(e / plane) % g.n3;
   const size_t r = e % plane;
   const int iky = (int)(r / g.sp), kx = (int)(r % g.sp);
   double2 v = zc();
@@ -388,8 +389,8 @@ __global__ __launch_bounds__(NT) void k_cz(Dev d, const double2* __restrict__ A,
   const Geo& g = d.g;
   const size_t plane = (size_t)g.ikty * g.sp;
   const size_t e = (size_t)blockIdx.x * NT + threadIdx.x;
-  if (e >= plane * g.n3) return;
-  const int z = (int)(e / plane);
+  if (e >= plane * g.nz) return;
+  const int z = (int)(e / plane) % g.n3;
   double2 v = zc();
   if (z < g.n3 - 1) {
     const double2 a1 = A[e + plane], a0 = A[e];
@@ -410,7 +411,7 @@ __global__ __launch_bounds__(NT) void k_diag(Dev d) {
     const int z = e / per, r = e % per, kx = r % g.KX, iky = box_row(r / g.KX, g);
     if (lmask(kx, iky, g)) {
       const size_t plane = (size_t)g.ikty * g.sp;
-      const size_t idx = z * plane + (size_t)iky * g.sp + kx;
+      const size_t idx = ((size_t)blockIdx.y * g.n3 + z) * plane + (size_t)iky * g.sp + kx;
       const int ky = ky_of(iky, g.n2);
       const double kh2 = (double)(kx * kx + ky * ky);
       const double w = (kx == 0 && iky == 0) ? 0.5 : 1.0;
@@ -455,7 +456,7 @@ __global__ __launch_bounds__(NT) void k_diag(Dev d) {
     }
     __syncthreads();
   }
-  if (threadIdx.x < 6) d.diag_part[(size_t)blockIdx.x * 6 + threadIdx.x] = red[threadIdx.x][0];
+  if (threadIdx.x < 6) d.diag_part[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * 6 + threadIdx.x] = red[threadIdx.x][0];
 }
 
 // ---------------------------------------------------------------------------
@@ -577,7 +578,7 @@ hipError_t launch_c2r(const Dev& d, hipStream_t st, const double2* k, double* r,
 template <int M, int P, int LPC>
 static hipError_t solve_c(const Dev& d, hipStream_t st, int f_lo, int nf, const double2* rhs_psi) {
   const int ncols = d.g.KX * d.g.nbox;
-  dim3 grid((ncols + LPC - 1) / LPC, nf);
+  dim3 grid((ncols + LPC - 1) / LPC, nf * d.g.nm);
   hipLaunchKernelGGL((k_solve_c<M, P, LPC>), grid, dim3(LPC * P), 0, st, d, f_lo, rhs_psi);
   return hipGetLastError();
 }
@@ -625,27 +626,27 @@ hipError_t launch_solve(const Dev& d, hipStream_t st, int f_lo, int f_hi, const 
   }
   // more than 512 levels: one sequential Thomas sweep per column
   const int ncols = d.g.KX * d.g.nbox;
-  dim3 grid((2 * ncols + NT - 1) / NT, nf);
+  dim3 grid((2 * ncols + NT - 1) / NT, nf * d.g.nm);
   hipLaunchKernelGGL(k_solve, grid, dim3(NT), 0, st, d, f_lo, rhs_psi);
   return hipGetLastError();
 }
 hipError_t launch_dq(const Dev& d, hipStream_t st) {
-  const int n = d.g.KX * d.g.nbox * d.g.n3;
+  const int n = d.g.KX * d.g.nbox * d.g.nz;
   hipLaunchKernelGGL(k_dq, dim3((n + NT - 1) / NT), dim3(NT), 0, st, d);
   return hipGetLastError();
 }
 hipError_t launch_clear_outside(const Dev& d, hipStream_t st, double2* f, int all) {
-  const size_t n = (size_t)d.g.ikty * d.g.sp * d.g.n3;
+  const size_t n = (size_t)d.g.ikty * d.g.sp * d.g.nz;
   hipLaunchKernelGGL(k_clear, dim3((unsigned)((n + NT - 1) / NT)), dim3(NT), 0, st, d.g, f, all);
   return hipGetLastError();
 }
 hipError_t launch_init_q(const Dev& d, hipStream_t st, const double2* psi, double2* q) {
-  const size_t n = (size_t)d.g.ikty * d.g.sp * d.g.n3;
+  const size_t n = (size_t)d.g.ikty * d.g.sp * d.g.nz;
   hipLaunchKernelGGL(k_init_q, dim3((unsigned)((n + NT - 1) / NT)), dim3(NT), 0, st, d, psi, q);
   return hipGetLastError();
 }
 hipError_t launch_cz(const Dev& d, hipStream_t st, const double2* A, double2* C) {
-  const size_t n = (size_t)d.g.ikty * d.g.sp * d.g.n3;
+  const size_t n = (size_t)d.g.ikty * d.g.sp * d.g.nz;
   hipLaunchKernelGGL(k_cz, dim3((unsigned)((n + NT - 1) / NT)), dim3(NT), 0, st, d, A, C);
   return hipGetLastError();
 }
@@ -653,7 +654,7 @@ int diag_blocks(const Geo& g) { return (g.KX * g.nbox * g.n3 + NT - 1) / NT; }
 hipError_t launch_diag(const Dev& d, hipStream_t st, int* nblocks) {
   const int nb = diag_blocks(d.g);
   *nblocks = nb;
-  hipLaunchKernelGGL(k_diag, dim3(nb), dim3(NT), 0, st, d);
+  hipLaunchKernelGGL(k_diag, dim3(nb, d.g.nm), dim3(NT), 0, st, d);
   return hipGetLastError();
 }
 

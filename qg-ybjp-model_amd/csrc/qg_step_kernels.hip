@@ -447,13 +447,13 @@ struct LS {
   static hipError_t row_nl_e(const Dev& d, hipStream_t st, int Win) {
     constexpr int RP = SH<N, E>::template rp<WG>();
     const double scale = 1.0 / ((double)d.g.n1 * (double)d.g.n2);
-    hipLaunchKernelGGL((k_row_nl<N, E, RP, 2>), dim3(d.g.n2 * d.g.n3 / RP), dim3(RP * (N / E)), 0, st, d, Win, scale);
+    hipLaunchKernelGGL((k_row_nl<N, E, RP, 2>), dim3(d.g.n2 * d.g.nz / RP), dim3(RP * (N / E)), 0, st, d, Win, scale);
     return hipGetLastError();
   }
   template <int E>
   static hipError_t col_upd_e(const Dev& d, hipStream_t st, int euler, int has_dq) {
     using S = SH<N, E>;
-    dim3 grid((d.g.KX + S::TX - 1) / S::TX, d.g.n3, 3);
+    dim3 grid((d.g.KX + S::TX - 1) / S::TX, d.g.nz, 3);
     hipLaunchKernelGGL((k_col_upd<N, E>), grid, dim3(S::COLT), 0, st, d, euler, has_dq);
     return hipGetLastError();
   }
@@ -461,13 +461,13 @@ struct LS {
   static hipError_t row_qw_e(const Dev& d, hipStream_t st) {
     constexpr int RP = SH<N, E>::template rp<WG>();
     const double scale = 1.0 / ((double)d.g.n1 * (double)d.g.n2);
-    hipLaunchKernelGGL((k_row_qw<N, E, RP, 2>), dim3(d.g.n2 * d.g.n3 / RP), dim3(RP * (N / E)), 0, st, d, scale);
+    hipLaunchKernelGGL((k_row_qw<N, E, RP, 2>), dim3(d.g.n2 * d.g.nz / RP), dim3(RP * (N / E)), 0, st, d, scale);
     return hipGetLastError();
   }
   template <int E>
   static hipError_t col_qw_e(const Dev& d, hipStream_t st) {
     using S = SH<N, E>;
-    dim3 grid((d.g.KX + S::TX - 1) / S::TX, d.g.n3);
+    dim3 grid((d.g.KX + S::TX - 1) / S::TX, d.g.nz);
     hipLaunchKernelGGL((k_col_qw<N, E>), grid, dim3(S::COLT), 0, st, d);
     return hipGetLastError();
   }

@@ -27,9 +27,11 @@ KERNEL_NAMES = ["col_inv", "row_nl", "col_upd", "row_qw", "col_qw", "solve", "so
 
 # exported symbols declared in include/qgybj.h
 SYMBOLS = [
-    "qg_create", "qg_destroy", "qg_last_error", "qg_set_state", "qg_set_state_physical", "qg_get_state",
+    "qg_create", "qg_create_batch", "qg_members", "qg_destroy", "qg_last_error", "qg_set_state", "qg_set_state_physical", "qg_get_state",
     "qg_step", "qg_iter", "qg_diag", "qg_init_q", "qg_fft_r2c", "qg_fft_c2r", "qg_device_field", "qg_device_pitch", "qg_stream",
-    "qg_sync", "qg_timing", "qg_kernel_times", "qg_kernel_name", "qg_set_variants", "qg_get_variants", "qg_mask", "qg_box", "qg_version",
+    "qg_sync", "qg_timing", "qg_kernel_times", "qg_kernel_name", "qg_set_variants", "qg_get_variants",
+    "qg_ensemble_create", "qg_ensemble_destroy", "qg_ensemble_error", "qg_ensemble_set_graph_steps",
+    "qg_ensemble_step", "qg_tune", "qg_mask", "qg_box", "qg_version",
 ]
 
 
@@ -63,6 +65,8 @@ def load(path: str = LIB_PATH):
     dp = C.POINTER(C.c_double)
     vp = C.c_void_p
     lib.qg_create.argtypes = [C.POINTER(_Params), C.POINTER(vp)]
+    lib.qg_create_batch.argtypes = [C.POINTER(_Params), C.c_int32, C.POINTER(vp)]
+    lib.qg_members.argtypes = [vp, C.POINTER(C.c_int32)]
     lib.qg_destroy.argtypes = [vp]
     lib.qg_last_error.argtypes = [vp]
     lib.qg_last_error.restype = C.c_char_p
@@ -85,6 +89,13 @@ def load(path: str = LIB_PATH):
     lib.qg_kernel_name.restype = C.c_char_p
     lib.qg_set_variants.argtypes = [vp, C.POINTER(C.c_int32), C.c_int32]
     lib.qg_get_variants.argtypes = [vp, C.POINTER(C.c_int32), C.c_int32]
+    lib.qg_ensemble_create.argtypes = [C.POINTER(vp), C.c_int32, C.POINTER(vp)]
+    lib.qg_ensemble_destroy.argtypes = [vp]
+    lib.qg_ensemble_error.argtypes = [vp]
+    lib.qg_ensemble_error.restype = C.c_char_p
+    lib.qg_ensemble_set_graph_steps.argtypes = [vp, C.c_int32]
+    lib.qg_ensemble_step.argtypes = [vp, C.c_int32]
+    lib.qg_tune.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32), dp]
     lib.qg_mask.argtypes = [C.c_int32, C.c_int32, C.POINTER(C.c_int32)]
     lib.qg_box.argtypes = [C.c_int32, C.c_int32, C.POINTER(C.c_int32)]
     lib.qg_version.restype = C.c_char_p
@@ -116,23 +127,28 @@ def box(n1: int):
 
 
 class Member:
-    """One QG-YBJ+ member on the current HIP device (a qg_ctx)."""
+    """One QG-YBJ+ member, or a batch of members with the same parameters,
+    on the current HIP device (a qg_ctx).  With members > 1 every array gets a
+    leading member axis."""
 
     FIELDS = ["psik", "qk", "qok", "BRk", "BIk", "BRok", "BIok", "ARk", "AIk"]
 
-    def __init__(self, P: QGParams):
+    def __init__(self, P: QGParams, members: int = 1):
         self.lib = load()
         self.P = P
+        self.members = int(members)
         self.n1, self.n3 = P.n1, P.n3
         self.iktx = P.n1 // 2 + 1
-        self.shape = (P.n3, P.n1, self.iktx)
+        lead = () if self.members == 1 else (self.members,)
+        self.shape = lead + (P.n3, P.n1, self.iktx)
+        self.rshape = lead + (P.n3, P.n1, P.n1)
         self._keep = [np.ascontiguousarray(a, dtype=np.float64) for a in (P.ifac_q, P.ifac_w, P.N2u, P.N2s)]
         prm = _Params(P.n1, P.n1, P.n3, P.flags, P.dt, P.gamma, P.Ro, P.Bu, P.W2F, P.nuz,
                       *[_dp(a) for a in self._keep])
         self._h = C.c_void_p()
-        rc = self.lib.qg_create(C.byref(prm), C.byref(self._h))
+        rc = self.lib.qg_create_batch(C.byref(prm), self.members, C.byref(self._h))
         if rc:
-            raise QGError(rc, "qg_create failed")
+            raise QGError(rc, "qg_create_batch failed")
 
     def _chk(self, rc, what):
         if rc:
@@ -164,7 +180,7 @@ class Member:
     def set_state_physical(self, psi_r, BR_r, BI_r):
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (psi_r, BR_r, BI_r)]
         for a in arrs:
-            assert a.shape == (self.n3, self.n1, self.n1)
+            assert a.shape == self.rshape, (a.shape, self.rshape)
         self._chk(self.lib.qg_set_state_physical(self._h, *[_dp(a) for a in arrs], self.n1),
                   "qg_set_state_physical")
 
@@ -189,9 +205,10 @@ class Member:
         return v.value
 
     def diag(self) -> np.ndarray:
-        out = np.zeros(6)
+        """[KE, PE, WKE, WPE, WCE, ENS] (per member: shape (members, 6))."""
+        out = np.zeros((self.members, 6))
         self._chk(self.lib.qg_diag(self._h, _dp(out)), "qg_diag")
-        return out
+        return out[0] if self.members == 1 else out
 
     def init_q(self, psik):
         psik = self._c(psik)
@@ -202,6 +219,7 @@ class Member:
     def r2c(self, r):
         r = np.ascontiguousarray(r, dtype=np.float64)
         nz = r.shape[0]
+        assert r.ndim == 3
         k = np.empty((nz, self.n1, self.iktx), np.complex128)
         self._chk(self.lib.qg_fft_r2c(self._h, _dp(r), r.shape[-1], _dp(k), nz), "qg_fft_r2c")
         return k
@@ -230,6 +248,15 @@ class Member:
     def sync(self):
         self._chk(self.lib.qg_sync(self._h), "qg_sync")
 
+    def tune(self, steps: int = 5):
+        """Run the tuner (qg_tune) from the current state; returns the chosen
+        variants and the best time per step in ms.  The state is unchanged."""
+        best = np.zeros(NVARIANTS, dtype=np.int32)
+        ms = C.c_double()
+        self._chk(self.lib.qg_tune(self._h, int(steps), best.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(ms)),
+                  "qg_tune")
+        return best, ms.value
+
     def timing(self, on=True):
         """on: True (every kernel), False, a kernel name or an id bit mask."""
         if on is True:
@@ -249,3 +276,38 @@ class Member:
         self._chk(self.lib.qg_kernel_times(self._h, _dp(ms), cnt.ctypes.data_as(C.POINTER(C.c_int32)), n),
                   "qg_kernel_times")
         return {KERNEL_NAMES[i]: (float(ms[i]), int(cnt[i])) for i in range(n)}
+
+
+class Ensemble:
+    """Contexts (members or batches) advanced together from captured HIP
+    graphs (qg_ensemble_*)."""
+
+    def __init__(self, members):
+        self.lib = load()
+        self.members = list(members)
+        arr = (C.c_void_p * len(self.members))(*[m._h.value for m in self.members])
+        self._h = C.c_void_p()
+        rc = self.lib.qg_ensemble_create(arr, len(self.members), C.byref(self._h))
+        if rc:
+            raise QGError(rc, "qg_ensemble_create failed")
+
+    def _chk(self, rc, what):
+        if rc:
+            raise QGError(rc, f"{what}: {self.lib.qg_ensemble_error(self._h).decode()}")
+
+    def set_graph_steps(self, k: int):
+        self._chk(self.lib.qg_ensemble_set_graph_steps(self._h, int(k)), "qg_ensemble_set_graph_steps")
+
+    def step(self, n: int = 1):
+        self._chk(self.lib.qg_ensemble_step(self._h, int(n)), "qg_ensemble_step")
+
+    def close(self):
+        if self._h:
+            self.lib.qg_ensemble_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
