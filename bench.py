@@ -60,6 +60,84 @@ def kernel_bytes(n1: int, n3: int, a_main: int = 0):
     }, {"G": G, "C": C, "KX": KX, "nbox": nbox}
 
 
+ENSEMBLE = [(128, 64, 16), (256, 64, 8), (256, 128, 4)]  # (n1 = n2, n3, members)
+
+
+def ensemble_bench(steps: int, warmup: int, tune: bool = True):
+    """Mixed-size ensemble: 16 members at 128^2 x 64, 8 at 256^2 x 64, 4 at
+    256^2 x 128, each with its own seed and a peak wavenumber drawn from 3..8.
+    Batched: one context per size class (a batch of its members), all classes
+    advanced together by one ensemble (graph replays).  Sequential: every
+    member alone in its own context, one after the other.  Both after the same
+    Euler step and warm-up, both with tuned kernel variants (qg_tune on the
+    batch and on one single member of each class)."""
+    import numpy as np
+    import qgybj
+    from qgybj import ic
+    from qgybj.params import make_params
+
+    rng = np.random.default_rng(20241)
+    classes = []
+    for n1, n3, nm in ENSEMBLE:
+        P = make_params(n1, n3, stratification="exponential")
+        k0s = rng.integers(3, 9, size=nm)
+        seeds = rng.integers(1, 2**31 - 1, size=nm)
+        ics = [(ic.eddy_psi(n1, n3, int(k), int(sd), Bu=P.Bu),) + ic.wave_b(n1, n3) for k, sd in zip(k0s, seeds)]
+        classes.append((P, nm, ics))
+    total = sum(nm for _, nm, _ in classes)
+
+    # sequential single-member runs
+    t_seq = 0.0
+    seq_variants = []
+    for P, nm, ics in classes:
+        var = None
+        for j in range(nm):
+            m = qgybj.Member(P)
+            m.set_state_physical(*ics[j])
+            if tune:
+                if var is None:
+                    var, _ = m.tune(3)
+                m.set_variants(var)
+            m.step(1 + warmup)
+            m.sync()
+            t0 = time.perf_counter()
+            m.step(steps)
+            t_seq += time.perf_counter() - t0
+            m.close()
+        seq_variants.append([int(x) for x in (var if var is not None else [])])
+
+    # batched: one batch per class, one ensemble
+    batches, bat_variants = [], []
+    for P, nm, ics in classes:
+        b = qgybj.Member(P, members=nm)
+        b.set_state_physical(*[np.stack([x[i] for x in ics]) for i in range(3)])
+        if tune:
+            v, _ = b.tune(3)
+            bat_variants.append([int(x) for x in v])
+        batches.append(b)
+    ens = qgybj.Ensemble(batches)
+    ens.step(1 + warmup)
+    t0 = time.perf_counter()
+    ens.step(steps)
+    t_bat = time.perf_counter() - t0
+    diag_ok = all(np.all(np.isfinite(b.diag())) for b in batches)
+    ens.close()
+    for b in batches:
+        b.close()
+    if not diag_ok:
+        raise RuntimeError("non-finite diagnostics in the ensemble")
+    seq = total * steps / t_seq
+    bat = total * steps / t_bat
+    return {"workload": "16 x 128^2x64 + 8 x 256^2x64 + 4 x 256^2x128 FP64, exponential N^2, seeds and "
+                        "peak |k| in 3..8 per member, feedback on",
+            "members": total, "steps": steps, "warmup": warmup,
+            "batched_member_steps_per_s": round(bat, 1), "sequential_member_steps_per_s": round(seq, 1),
+            "batched_over_sequential": round(bat / seq, 3),
+            "ms_per_ensemble_step_batched": round(1e3 * t_bat / steps, 4),
+            "ms_per_ensemble_step_sequential": round(1e3 * t_seq / steps, 4),
+            "variants_batched": bat_variants, "variants_single": seq_variants}
+
+
 def cpu_baseline(P, psi, BR, BI, nsteps: int = 2):
     """Time the oracle (NumPy/SciPy FP64 restatement) on nsteps leapfrog steps
     after an untimed Euler step."""
@@ -101,6 +179,8 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--ensemble-steps", type=int, default=50,
+                    help="timed steps of the mixed-size ensemble, batched and one member at a time (0: skip)")
     ap.add_argument("--profile-steps", type=int, default=20,
                     help="untimed steps with every kernel timed (kernel table, dominant kernel)")
     ap.add_argument("--cpu-steps", type=int, default=2)
@@ -175,6 +255,10 @@ def main(argv=None):
     step_alg = sum(kb[k] * kt_all[k][1] for k in kb) / a.profile_steps
     m.close()
 
+    ens = None
+    if a.ensemble_steps > 0:
+        ens = ensemble_bench(a.ensemble_steps, 10)
+
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(P, psi, BR, BI, a.cpu_steps)
@@ -197,6 +281,7 @@ def main(argv=None):
                    "n1": N1, "n2": N1, "n3": N3, "members": 1, "variants": variants},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "ensemble": ens,
         "kernels": kernels,
         "step_alg_bytes": int(step_alg),
         "step_effective_GBs": round(step_alg / (ms_per_step * 1e-3) / 1e9, 1),
