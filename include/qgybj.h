@@ -197,6 +197,39 @@ int qg_ensemble_step(qg_ens* e, int32_t nsteps);
  * state.  No counterpart in the reference. */
 int qg_tune(qg_ctx* ctx, int32_t steps, int32_t* best, double* best_ms);
 
+/* Tangent-linear and adjoint of the step, for gradients with respect to the
+ * initial streamfunction.  An adjoint context is bound to one single-member
+ * context with nuz = 0 and flags 0 or QG_NO_FEEDBACK, whose state it uses as
+ * the forward trajectory (the member is left at the final state of the last
+ * run).  Objective: J = 1/2 * (2 pi)^2 dz * sum over the levels with
+ * z_s < z_deep of sum over L = 1 modes of W kh2 (|AR|^2 + |AI|^2) after
+ * nsteps, i.e. the domain integral of |grad_h A|^2 / 2 below the depth
+ * L3 - z_deep (W: 2 per half-spectrum mode, 1 at kx = ky = 0).  Controls and
+ * gradients are physical fields (ldx, n2, n3) like qg_set_state_physical's;
+ * the forward run starts from psir, BRr, BIr with A = 0.  The adjoint sweep
+ * recomputes the forward trajectory from checkpoints stored every
+ * checkpoint_every steps.  No counterpart in the reference.
+ *  qg_adj_objective: J only.
+ *  qg_adj_tlm:       directional derivative dJ along dpsir, and (if dstate is
+ *                    not NULL) the perturbation of the final state, 9 spectral
+ *                    fields (psi, q, qo, BR, BI, BRo, BIo, AR, AI) in the
+ *                    qg_get_state layout, one after the other.
+ *  qg_adj_apply:     transpose of that map: the physical psi adjoint of a
+ *                    given final-state adjoint (same 9-field layout).
+ *  qg_adj_gradient:  J and dJ/dpsir. */
+typedef struct qg_adj qg_adj;
+int qg_adj_create(qg_ctx* ctx, int32_t checkpoint_every, double z_deep, qg_adj** out);
+int qg_adj_destroy(qg_adj* a);
+const char* qg_adj_error(const qg_adj* a);
+int qg_adj_objective(qg_adj* a, const double* psir, const double* BRr, const double* BIr, int32_t ldx,
+                     int32_t nsteps, double* J);
+int qg_adj_tlm(qg_adj* a, const double* psir, const double* BRr, const double* BIr, int32_t ldx, int32_t nsteps,
+               const double* dpsir, double* dJ, double* dstate);
+int qg_adj_apply(qg_adj* a, const double* psir, const double* BRr, const double* BIr, int32_t ldx, int32_t nsteps,
+                 const double* state_bar, double* psi_bar);
+int qg_adj_gradient(qg_adj* a, const double* psir, const double* BRr, const double* BIr, int32_t ldx,
+                    int32_t nsteps, double* J, double* grad);
+
 /* Host-only helpers (no GPU needed): the dealiasing mask L(iktx, ikty) as the
  * library evaluates it, and the box geometry {kmax, KX, nlo, nhi}. */
 int qg_mask(int32_t n1, int32_t n2, int32_t* L);

@@ -1,0 +1,46 @@
+// Pointwise kernels of the tangent-linear and adjoint steps (qg_adj_kernels.hip).
+// Internal; not part of the public interface (include/qgybj.h).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+#include "qg_common.hpp"
+
+namespace qg {
+
+enum : int { TAB_NONE = 0, TAB_E1Q, TAB_E2Q, TAB_E1W, TAB_E2W };
+
+// one term of a spectral combination: coefficient
+//   a * (c0 + ckx kx + cky ky + ckh kh2) * (i if imag) * W^wp * table(kx, ky)
+struct SMulTerm {
+  const double2* in = nullptr;
+  double a = 1.0, c0 = 1.0, ckx = 0.0, cky = 0.0, ckh = 0.0;
+  int imag = 0, wp = 0, tab = TAB_NONE;
+};
+constexpr int SMUL_TERMS = 4;
+// out = (accumulate ? out : 0) + sum of the terms, at L = 1 modes of levels
+// [zlo, zhi) of every member; elsewhere out = 0 unless accumulate (then
+// unchanged, or 0 with clear_off)
+struct SMulArgs {
+  SMulTerm t[SMUL_TERMS];
+  int zlo = 0, zhi = 1 << 30;
+  int accumulate = 0, clear_off = 0;
+};
+
+constexpr int PFMA_TERMS = 4;
+// out = (accumulate ? out : 0) + sum_j s_j * x_j * (y_j or 1)
+struct PFmaArgs {
+  const double* x[PFMA_TERMS] = {nullptr, nullptr, nullptr, nullptr};
+  const double* y[PFMA_TERMS] = {nullptr, nullptr, nullptr, nullptr};
+  double s[PFMA_TERMS] = {1.0, 1.0, 1.0, 1.0};
+  int accumulate = 0;
+};
+
+hipError_t launch_smul(const Dev& d, hipStream_t st, double2* out, const SMulArgs& a);
+hipError_t launch_pfma(hipStream_t st, double* out, size_t n, const PFmaArgs& a);
+int obj_blocks(const Geo& g);
+hipError_t launch_obj(const Dev& d, hipStream_t st, const double2* AR, const double2* AI, int zlo, int zhi,
+                      double* part);
+
+}  // namespace qg

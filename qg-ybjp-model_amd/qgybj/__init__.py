@@ -15,7 +15,7 @@ import os
 
 import numpy as np
 
-from .params import QGParams, make_params  # noqa: F401
+from .params import TWOPI, QGParams, make_params  # noqa: F401
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libqgybj.so")
@@ -31,7 +31,8 @@ SYMBOLS = [
     "qg_step", "qg_iter", "qg_diag", "qg_init_q", "qg_fft_r2c", "qg_fft_c2r", "qg_device_field", "qg_device_pitch", "qg_stream",
     "qg_sync", "qg_timing", "qg_kernel_times", "qg_kernel_name", "qg_set_variants", "qg_get_variants",
     "qg_ensemble_create", "qg_ensemble_destroy", "qg_ensemble_error", "qg_ensemble_set_graph_steps",
-    "qg_ensemble_step", "qg_tune", "qg_mask", "qg_box", "qg_version",
+    "qg_ensemble_step", "qg_tune", "qg_adj_create", "qg_adj_destroy", "qg_adj_error", "qg_adj_objective",
+    "qg_adj_tlm", "qg_adj_apply", "qg_adj_gradient", "qg_mask", "qg_box", "qg_version",
 ]
 
 
@@ -96,6 +97,14 @@ def load(path: str = LIB_PATH):
     lib.qg_ensemble_set_graph_steps.argtypes = [vp, C.c_int32]
     lib.qg_ensemble_step.argtypes = [vp, C.c_int32]
     lib.qg_tune.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32), dp]
+    lib.qg_adj_create.argtypes = [vp, C.c_int32, C.c_double, C.POINTER(vp)]
+    lib.qg_adj_destroy.argtypes = [vp]
+    lib.qg_adj_error.argtypes = [vp]
+    lib.qg_adj_error.restype = C.c_char_p
+    lib.qg_adj_objective.argtypes = [vp, dp, dp, dp, C.c_int32, C.c_int32, dp]
+    lib.qg_adj_tlm.argtypes = [vp, dp, dp, dp, C.c_int32, C.c_int32, dp, dp, dp]
+    lib.qg_adj_apply.argtypes = [vp, dp, dp, dp, C.c_int32, C.c_int32, dp, dp]
+    lib.qg_adj_gradient.argtypes = [vp, dp, dp, dp, C.c_int32, C.c_int32, dp, dp]
     lib.qg_mask.argtypes = [C.c_int32, C.c_int32, C.POINTER(C.c_int32)]
     lib.qg_box.argtypes = [C.c_int32, C.c_int32, C.POINTER(C.c_int32)]
     lib.qg_version.restype = C.c_char_p
@@ -304,6 +313,78 @@ class Ensemble:
     def close(self):
         if self._h:
             self.lib.qg_ensemble_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Adjoint:
+    """Tangent-linear and adjoint of the step bound to a single-member context
+    (qg_adj_*).  Objective: domain integral of |grad_h A|^2 / 2 over the
+    levels with z_s < z_deep after nsteps; control: the physical initial
+    streamfunction psi (n3, n1, n1)."""
+
+    def __init__(self, member: "Member", checkpoint_every: int = 20, z_deep: float | None = None):
+        self.lib = load()
+        self.m = member
+        self.z_deep = 0.8 * TWOPI if z_deep is None else float(z_deep)
+        self._h = C.c_void_p()
+        rc = self.lib.qg_adj_create(member._h, int(checkpoint_every), self.z_deep, C.byref(self._h))
+        if rc:
+            raise QGError(rc, f"qg_adj_create: {member.lib.qg_last_error(member._h).decode()}")
+
+    def _chk(self, rc, what):
+        if rc:
+            raise QGError(rc, f"{what}: {self.lib.qg_adj_error(self._h).decode()}")
+
+    def _phys(self, *arrs):
+        out = [np.ascontiguousarray(a, dtype=np.float64) for a in arrs]
+        for a in out:
+            assert a.shape == self.m.rshape, (a.shape, self.m.rshape)
+        return out
+
+    def objective(self, psi, BR, BI, nsteps: int) -> float:
+        psi, BR, BI = self._phys(psi, BR, BI)
+        J = C.c_double()
+        self._chk(self.lib.qg_adj_objective(self._h, _dp(psi), _dp(BR), _dp(BI), self.m.n1, int(nsteps), C.byref(J)),
+                  "qg_adj_objective")
+        return J.value
+
+    def tlm(self, psi, BR, BI, nsteps: int, dpsi, want_state: bool = False):
+        """Returns dJ along dpsi and (optionally) the final-state perturbation
+        as a (9, n3, n1, iktx) complex array (psi, q, qo, BR, BI, BRo, BIo, AR, AI)."""
+        psi, BR, BI, dpsi = self._phys(psi, BR, BI, dpsi)
+        dJ = C.c_double()
+        st = np.empty((9,) + self.m.shape, np.complex128) if want_state else None
+        self._chk(self.lib.qg_adj_tlm(self._h, _dp(psi), _dp(BR), _dp(BI), self.m.n1, int(nsteps), _dp(dpsi),
+                                      C.byref(dJ), _dp(st)), "qg_adj_tlm")
+        return dJ.value, st
+
+    def apply(self, psi, BR, BI, nsteps: int, state_bar):
+        """Adjoint of the map psi -> final state, applied to state_bar (9, n3, n1, iktx)."""
+        psi, BR, BI = self._phys(psi, BR, BI)
+        sb = np.ascontiguousarray(state_bar, dtype=np.complex128)
+        assert sb.shape == (9,) + self.m.shape
+        out = np.empty(self.m.rshape, np.float64)
+        self._chk(self.lib.qg_adj_apply(self._h, _dp(psi), _dp(BR), _dp(BI), self.m.n1, int(nsteps), _dp(sb),
+                                        _dp(out)), "qg_adj_apply")
+        return out
+
+    def gradient(self, psi, BR, BI, nsteps: int):
+        psi, BR, BI = self._phys(psi, BR, BI)
+        J = C.c_double()
+        g = np.empty(self.m.rshape, np.float64)
+        self._chk(self.lib.qg_adj_gradient(self._h, _dp(psi), _dp(BR), _dp(BI), self.m.n1, int(nsteps),
+                                           C.byref(J), _dp(g)), "qg_adj_gradient")
+        return J.value, g
+
+    def close(self):
+        if self._h:
+            self.lib.qg_adj_destroy(self._h)
             self._h = C.c_void_p()
 
     def __del__(self):
