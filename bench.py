@@ -138,6 +138,43 @@ def ensemble_bench(steps: int, warmup: int, tune: bool = True):
             "variants_batched": bat_variants, "variants_single": seq_variants}
 
 
+def adjoint_bench(steps: int = 200, every: int = 20, repeats: int = 2):
+    """Gradient of the deep wave energy (domain integral of |grad A|^2/2 below
+    depth 0.2 L3) after `steps` steps at 128^2 x 64 with respect to the initial
+    psi, checkpoints every `every` steps: wall time of qg_adj_gradient over
+    that of the forward run alone (qg_adj_objective, same upload and
+    objective), best of `repeats`."""
+    import numpy as np
+    import qgybj
+    from qgybj import ic
+    from qgybj.params import make_params
+
+    P = make_params(128, 64, stratification="exponential")
+    psi = ic.eddy_psi(128, 64, 4, 3, Bu=P.Bu)
+    BR, BI = ic.wave_b(128, 64)
+    m = qgybj.Member(P)
+    adj = qgybj.Adjoint(m, checkpoint_every=every)
+    adj.objective(psi, BR, BI, 2)  # warm-up
+    t_f, t_g = [], []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        J = adj.objective(psi, BR, BI, steps)
+        t_f.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        J2, g = adj.gradient(psi, BR, BI, steps)
+        t_g.append(time.perf_counter() - t0)
+    ok = bool(np.isfinite(J) and J == J2 and np.all(np.isfinite(g)))
+    adj.close()
+    m.close()
+    if not ok:
+        raise RuntimeError("adjoint benchmark: inconsistent objective or non-finite gradient")
+    tf, tg = min(t_f), min(t_g)
+    return {"workload": f"128^2x64 FP64, exponential N^2, {steps} steps, checkpoints every {every}, "
+                        "objective: deep wave energy, control: initial psi",
+            "forward_s": round(tf, 4), "gradient_s": round(tg, 4), "adjoint_over_forward": round(tg / tf, 3),
+            "J": J}
+
+
 def cpu_baseline(P, psi, BR, BI, nsteps: int = 2):
     """Time the oracle (NumPy/SciPy FP64 restatement) on nsteps leapfrog steps
     after an untimed Euler step."""
@@ -181,6 +218,8 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--ensemble-steps", type=int, default=50,
                     help="timed steps of the mixed-size ensemble, batched and one member at a time (0: skip)")
+    ap.add_argument("--adjoint-steps", type=int, default=200,
+                    help="steps of the adjoint-gradient leg at 128^2 x 64 (0: skip)")
     ap.add_argument("--profile-steps", type=int, default=20,
                     help="untimed steps with every kernel timed (kernel table, dominant kernel)")
     ap.add_argument("--cpu-steps", type=int, default=2)
@@ -258,6 +297,9 @@ def main(argv=None):
     ens = None
     if a.ensemble_steps > 0:
         ens = ensemble_bench(a.ensemble_steps, 10)
+    adjr = None
+    if a.adjoint_steps > 0:
+        adjr = adjoint_bench(a.adjoint_steps, 20)
 
     cpu = None
     if not a.no_cpu:
@@ -282,6 +324,7 @@ def main(argv=None):
         "roofline": roofline,
         "cpu_baseline": cpu,
         "ensemble": ens,
+        "adjoint": adjr,
         "kernels": kernels,
         "step_alg_bytes": int(step_alg),
         "step_effective_GBs": round(step_alg / (ms_per_step * 1e-3) / 1e9, 1),

@@ -221,6 +221,11 @@ typedef struct qg_adj qg_adj;
 int qg_adj_create(qg_ctx* ctx, int32_t checkpoint_every, double z_deep, qg_adj** out);
 int qg_adj_destroy(qg_adj* a);
 const char* qg_adj_error(const qg_adj* a);
+/* 1 (default): the adjoint step runs on the fused transform kernels; 0: on
+ * generic pointwise kernels and separate transforms (the reference form the
+ * fused kernels are tested against).  Both give the same results to FP64
+ * rounding. */
+int qg_adj_set_fused(qg_adj* a, int32_t fused);
 int qg_adj_objective(qg_adj* a, const double* psir, const double* BRr, const double* BIr, int32_t ldx,
                      int32_t nsteps, double* J);
 int qg_adj_tlm(qg_adj* a, const double* psir, const double* BRr, const double* BIr, int32_t ldx, int32_t nsteps,

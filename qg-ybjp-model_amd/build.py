@@ -35,7 +35,7 @@ HIP_FLAGS = [
     "-I", INC, "-I", CSRC,
 ]
 
-SOURCES = ["qg_step_kernels.hip", "qg_kernels.hip", "qg_adj_kernels.hip", "qg_api.cpp", "qg_ensemble.cpp", "qg_adj.cpp"]
+SOURCES = ["qg_step_kernels.hip", "qg_kernels.hip", "qg_adj_kernels.hip", "qg_adj_step.hip", "qg_api.cpp", "qg_ensemble.cpp", "qg_adj.cpp"]
 
 
 def _run(cmd, verbose):

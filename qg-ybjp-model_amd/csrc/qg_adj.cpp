@@ -60,6 +60,9 @@ struct qg_adj {
   double2* x[NSTATE] = {};
   double2* s[12] = {};
   double* p[24] = {};
+  double2* F[6] = {};  // fused adjoint: forward (kx, y) planes
+  double2* A[6] = {};  // fused adjoint: adjoint (kx, y) planes
+  int fused = 1;       // 1: fused adjoint kernels, 0: generic kernels
   double* part = nullptr;
   // trajectory storage: checkpoints (full member state) and one segment
   std::vector<double2*> ckpt;     // 12 fields each
@@ -429,6 +432,48 @@ int adj_step(qg_adj* a, const double2* const* fn, const double2* BR1, const doub
   return QG_OK;
 }
 
+// ---- fused adjoint step -------------------------------------------------------
+// Same map as adj_step, with the transform work in the fused kernels of
+// qg_adj_step.hip; the forward (kx, y) planes are recomputed from the stored
+// spectral states with the step's own y-inverse kernel.
+int adj_step_fused(qg_adj* a, const double2* const* fn, const double2* BR1, const double2* BI1, bool euler) {
+  Ops o{a};
+  const Coef k = coefs(a);
+  const Dev& d = a->c->d;
+  const auto& g = d.g;
+  double2** x = a->x;
+  double2** s = a->s;
+  qg::AdjBufs b;
+  for (int i = 0; i < 6; ++i) {
+    b.F[i] = a->F[i];
+    b.A[i] = a->A[i];
+  }
+  b.psi = x[F_PSI]; b.q = x[F_Q]; b.qo = x[F_QO]; b.BR = x[F_BR]; b.BI = x[F_BI];
+  b.BRo = x[F_BRO]; b.BIo = x[F_BIO]; b.AR = x[F_AR]; b.AI = x[F_AI];
+  b.qstar = s[8];
+  b.sR = s[6];
+  b.sI = s[7];
+  ARC(o.solve_psi(x[F_PSI], s[8]));
+  ARC(o.solve_a(x[F_AR], s[6], x[F_AI], s[7]));
+  if (k.feedback) {
+    AHIP(a, qg::launch_adj_col_qw(d, a->st, b));
+    AHIP(a, qg::launch_col_inv(d, a->st, BR1, a->F[2], a->F[3], nullptr, g.KX, 1, 1, g.nz));
+    AHIP(a, qg::launch_col_inv(d, a->st, BI1, a->F[4], a->F[5], nullptr, g.KX, 1, 1, g.nz));
+    AHIP(a, qg::launch_adj_row_qw(d, a->st, b));
+  }
+  AHIP(a, qg::launch_adj_col_upd(d, a->st, b, euler ? 1 : 0, k.feedback ? 1 : 0));
+  // forward (kx, y) planes of step n as row_nl reads them (the Euler step's
+  // fields are not yet box-restricted)
+  const int Win = euler ? g.iktx : g.KX;
+  AHIP(a, qg::launch_col_inv(d, a->st, fn[0], a->F[1], a->F[3], a->F[5], g.KX, 1, 1, g.nz));
+  AHIP(a, qg::launch_col_inv(d, a->st, fn[1], a->F[0], nullptr, nullptr, Win, euler ? 0 : 1, 0, g.nz));
+  AHIP(a, qg::launch_col_inv(d, a->st, fn[2], a->F[2], nullptr, nullptr, Win, euler ? 0 : 1, 0, g.nz));
+  AHIP(a, qg::launch_col_inv(d, a->st, fn[3], a->F[4], nullptr, nullptr, Win, euler ? 0 : 1, 0, g.nz));
+  AHIP(a, qg::launch_adj_row_nl(d, a->st, b, Win, euler ? 1 : 0));
+  AHIP(a, qg::launch_adj_col_out(d, a->st, b));
+  return QG_OK;
+}
+
 // ---- trajectory ---------------------------------------------------------------
 double2* const* member_fields12(qg_ctx* c, double2** out) {
   Dev& d = c->d;
@@ -535,7 +580,10 @@ int sweep(qg_adj* a, int nsteps) {
     }
     for (int j = len - 1; j >= 0; --j) {
       const double2* fn[4] = {seg_field(a, j, 0), seg_field(a, j, 1), seg_field(a, j, 2), seg_field(a, j, 3)};
-      ARC(adj_step(a, fn, seg_field(a, j + 1, 2), seg_field(a, j + 1, 3), n0 + j == 0));
+      if (a->fused)
+        ARC(adj_step_fused(a, fn, seg_field(a, j + 1, 2), seg_field(a, j + 1, 3), n0 + j == 0));
+      else
+        ARC(adj_step(a, fn, seg_field(a, j + 1, 2), seg_field(a, j + 1, 3), n0 + j == 0));
     }
   }
   return QG_OK;
@@ -613,6 +661,8 @@ int qg_adj_create(qg_ctx* c, int32_t checkpoint_every, double z_deep, qg_adj** o
   for (int i = 0; i < NSTATE && rc == QG_OK; ++i) rc = aalloc(a, &a->x[i], a->S);
   for (int i = 0; i < 12 && rc == QG_OK; ++i) rc = aalloc(a, &a->s[i], a->S);
   for (int i = 0; i < 24 && rc == QG_OK; ++i) rc = aalloc(a, &a->p[i], a->NP);
+  for (int i = 0; i < 6 && rc == QG_OK; ++i) rc = aalloc(a, &a->F[i], a->S);
+  for (int i = 0; i < 6 && rc == QG_OK; ++i) rc = aalloc(a, &a->A[i], a->S);
   double2* tq = nullptr;
   if (rc == QG_OK) rc = aalloc(a, &tq, a->S);
   a->w.qs = tq;  // transform temporary of the raw r2c / c2r
@@ -639,6 +689,12 @@ int qg_adj_destroy(qg_adj* a) {
 }
 
 const char* qg_adj_error(const qg_adj* a) { return a ? a->err.c_str() : "null adjoint context"; }
+
+int qg_adj_set_fused(qg_adj* a, int32_t fused) {
+  if (!a || fused < 0 || fused > 1) return QG_ERR_ARG;
+  a->fused = fused;
+  return QG_OK;
+}
 
 int qg_adj_objective(qg_adj* a, const double* psir, const double* BRr, const double* BIr, int32_t ldx,
                      int32_t nsteps, double* J) {

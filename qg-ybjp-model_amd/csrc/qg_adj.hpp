@@ -37,6 +37,22 @@ struct PFmaArgs {
   int accumulate = 0;
 };
 
+// buffers of the fused adjoint step (qg_adj_step.hip): forward and adjoint
+// half-transformed (kx, y) planes, the state adjoint and the inversion outputs
+struct AdjBufs {
+  double2* F[6];
+  double2* A[6];
+  double2 *psi, *q, *qo, *BR, *BI, *BRo, *BIo, *AR, *AI;
+  const double2* qstar;  // S_psi(psibar')
+  const double2* sR;     // S_A(ARbar')
+  const double2* sI;     // S_A(AIbar')
+};
+hipError_t launch_adj_col_qw(const Dev& d, hipStream_t st, const AdjBufs& b);
+hipError_t launch_adj_row_qw(const Dev& d, hipStream_t st, const AdjBufs& b);
+hipError_t launch_adj_col_upd(const Dev& d, hipStream_t st, const AdjBufs& b, int euler, int feedback);
+hipError_t launch_adj_row_nl(const Dev& d, hipStream_t st, const AdjBufs& b, int Win, int euler);
+hipError_t launch_adj_col_out(const Dev& d, hipStream_t st, const AdjBufs& b);
+
 hipError_t launch_smul(const Dev& d, hipStream_t st, double2* out, const SMulArgs& a);
 hipError_t launch_pfma(hipStream_t st, double* out, size_t n, const PFmaArgs& a);
 int obj_blocks(const Geo& g);

@@ -31,7 +31,7 @@ SYMBOLS = [
     "qg_step", "qg_iter", "qg_diag", "qg_init_q", "qg_fft_r2c", "qg_fft_c2r", "qg_device_field", "qg_device_pitch", "qg_stream",
     "qg_sync", "qg_timing", "qg_kernel_times", "qg_kernel_name", "qg_set_variants", "qg_get_variants",
     "qg_ensemble_create", "qg_ensemble_destroy", "qg_ensemble_error", "qg_ensemble_set_graph_steps",
-    "qg_ensemble_step", "qg_tune", "qg_adj_create", "qg_adj_destroy", "qg_adj_error", "qg_adj_objective",
+    "qg_ensemble_step", "qg_tune", "qg_adj_create", "qg_adj_destroy", "qg_adj_error", "qg_adj_set_fused", "qg_adj_objective",
     "qg_adj_tlm", "qg_adj_apply", "qg_adj_gradient", "qg_mask", "qg_box", "qg_version",
 ]
 
@@ -101,6 +101,7 @@ def load(path: str = LIB_PATH):
     lib.qg_adj_destroy.argtypes = [vp]
     lib.qg_adj_error.argtypes = [vp]
     lib.qg_adj_error.restype = C.c_char_p
+    lib.qg_adj_set_fused.argtypes = [vp, C.c_int32]
     lib.qg_adj_objective.argtypes = [vp, dp, dp, dp, C.c_int32, C.c_int32, dp]
     lib.qg_adj_tlm.argtypes = [vp, dp, dp, dp, C.c_int32, C.c_int32, dp, dp, dp]
     lib.qg_adj_apply.argtypes = [vp, dp, dp, dp, C.c_int32, C.c_int32, dp, dp]
@@ -346,6 +347,9 @@ class Adjoint:
         for a in out:
             assert a.shape == self.m.rshape, (a.shape, self.m.rshape)
         return out
+
+    def set_fused(self, fused: bool):
+        self._chk(self.lib.qg_adj_set_fused(self._h, 1 if fused else 0), "qg_adj_set_fused")
 
     def objective(self, psi, BR, BI, nsteps: int) -> float:
         psi, BR, BI = self._phys(psi, BR, BI)
