@@ -70,6 +70,7 @@ struct qg_adj {
   double2* seg = nullptr;         // (K + 1) x 4 fields: psi, q, BR, BI
   double2* fin = nullptr;         // final state (12 fields)
   int fin_meta[3] = {0, 0, 0};
+  int last_seg = -1;  // checkpoint index whose segment the forward run already stored
   std::string err;
 };
 
@@ -207,16 +208,24 @@ struct Ops {
   }
 };
 
-// physical fields of a stored state: u, v, zeta (from L psi), q, BR, BI
-// (as stored) -> p[0..5]
-int fwd_fields_n(const Ops& o, const double2* psi, const double2* q, const double2* BR, const double2* BI) {
+// physical fields of a stored state: u, v, zeta (from L psi), q, BR, BI ->
+// p[0..5].  q, BR, BI as stored before the Euler step (the initial B is not
+// dealiased), their L = 1 modes afterwards (stored slots keep only the box).
+int fwd_fields_n(const Ops& o, const double2* psi, const double2* q, const double2* BR, const double2* BI,
+                 bool euler) {
   qg_adj* a = o.a;
   ARC(o.c2r_of(a->p[0], {t_iky(psi, -1.0)}));
   ARC(o.c2r_of(a->p[1], {t_ikx(psi)}));
   ARC(o.c2r_of(a->p[2], {t_kh2(psi, -1.0)}));
-  ARC(o.c2r(q, a->p[3]));
-  ARC(o.c2r(BR, a->p[4]));
-  ARC(o.c2r(BI, a->p[5]));
+  if (euler) {
+    ARC(o.c2r(q, a->p[3]));
+    ARC(o.c2r(BR, a->p[4]));
+    ARC(o.c2r(BI, a->p[5]));
+  } else {
+    ARC(o.c2r_of(a->p[3], {term(q)}));
+    ARC(o.c2r_of(a->p[4], {term(BR)}));
+    ARC(o.c2r_of(a->p[5], {term(BI)}));
+  }
   return QG_OK;
 }
 // gradient fields of the new B for the feedback term: BRx, BRy, BIx, BIy,
@@ -227,8 +236,8 @@ int fwd_fields_w(const Ops& o, const double2* BR, const double2* BI) {
   ARC(o.c2r_of(a->p[7], {t_iky(BR)}));
   ARC(o.c2r_of(a->p[8], {t_ikx(BI)}));
   ARC(o.c2r_of(a->p[9], {t_iky(BI)}));
-  ARC(o.c2r(BR, a->p[10]));
-  ARC(o.c2r(BI, a->p[11]));
+  ARC(o.c2r_of(a->p[10], {term(BR)}));
+  ARC(o.c2r_of(a->p[11], {term(BI)}));
   return QG_OK;
 }
 
@@ -250,7 +259,7 @@ int tlm_step(qg_adj* a, const double2* const* fn, const double2* BR1, const doub
   double2** x = a->x;
   double2** s = a->s;
   double** p = a->p;
-  ARC(fwd_fields_n(o, fn[0], fn[1], fn[2], fn[3]));
+  ARC(fwd_fields_n(o, fn[0], fn[1], fn[2], fn[3], euler));
   // perturbation fields p[12..17]: du, dv, dzeta, dq, dBR, dBI
   ARC(o.c2r_of(p[12], {t_iky(x[F_PSI], -1.0)}));
   ARC(o.c2r_of(p[13], {t_ikx(x[F_PSI])}));
@@ -401,7 +410,7 @@ int adj_step(qg_adj* a, const double2* const* fn, const double2* BR1, const doub
   }
   // 6. nonlinear terms: physical adjoints p[12..17] = ubar, vbar, zetabar,
   // qbar, BRbar, BIbar
-  ARC(fwd_fields_n(o, fn[0], fn[1], fn[2], fn[3]));
+  ARC(fwd_fields_n(o, fn[0], fn[1], fn[2], fn[3], euler));
   const double2* nX[3] = {nq, nR, nI};
   for (int j = 0; j < 3; ++j) {
     // a_x = N^-1 c2r(-i kx nX / W), a_y = N^-1 c2r(-i ky nX / W)
@@ -453,12 +462,20 @@ int adj_step_fused(qg_adj* a, const double2* const* fn, const double2* BR1, cons
   b.qstar = s[8];
   b.sR = s[6];
   b.sI = s[7];
-  ARC(o.solve_psi(x[F_PSI], s[8]));
-  ARC(o.solve_a(x[F_AR], s[6], x[F_AI], s[7]));
+  {
+    // the three inversions in one launch: psi from psibar', A from ARbar', AIbar'
+    Dev dd = a->c->d;
+    dd.psi = s[8];
+    dd.BR = x[F_AR];
+    dd.AR = s[6];
+    dd.BI = x[F_AI];
+    dd.AI = s[7];
+    AHIP(a, qg::launch_solve(dd, a->st, 0, 2, x[F_PSI]));
+  }
   if (k.feedback) {
     AHIP(a, qg::launch_adj_col_qw(d, a->st, b));
-    AHIP(a, qg::launch_col_inv(d, a->st, BR1, a->F[2], a->F[3], nullptr, g.KX, 1, 1, g.nz));
-    AHIP(a, qg::launch_col_inv(d, a->st, BI1, a->F[4], a->F[5], nullptr, g.KX, 1, 1, g.nz));
+    // BR', BI' are adjacent in the segment slot, and so are F2, F4 and F3, F5
+    AHIP(a, qg::launch_col_inv(d, a->st, BR1, a->F[2], a->F[3], nullptr, g.KX, 1, 1, 2 * g.nz));
     AHIP(a, qg::launch_adj_row_qw(d, a->st, b));
   }
   AHIP(a, qg::launch_adj_col_upd(d, a->st, b, euler ? 1 : 0, k.feedback ? 1 : 0));
@@ -466,9 +483,8 @@ int adj_step_fused(qg_adj* a, const double2* const* fn, const double2* BR1, cons
   // fields are not yet box-restricted)
   const int Win = euler ? g.iktx : g.KX;
   AHIP(a, qg::launch_col_inv(d, a->st, fn[0], a->F[1], a->F[3], a->F[5], g.KX, 1, 1, g.nz));
-  AHIP(a, qg::launch_col_inv(d, a->st, fn[1], a->F[0], nullptr, nullptr, Win, euler ? 0 : 1, 0, g.nz));
-  AHIP(a, qg::launch_col_inv(d, a->st, fn[2], a->F[2], nullptr, nullptr, Win, euler ? 0 : 1, 0, g.nz));
-  AHIP(a, qg::launch_col_inv(d, a->st, fn[3], a->F[4], nullptr, nullptr, Win, euler ? 0 : 1, 0, g.nz));
+  // q, BR, BI are adjacent in the segment slot, and so are F0, F2, F4
+  AHIP(a, qg::launch_col_inv(d, a->st, fn[1], a->F[0], nullptr, nullptr, Win, euler ? 0 : 1, 0, 3 * g.nz));
   AHIP(a, qg::launch_adj_row_nl(d, a->st, b, Win, euler ? 1 : 0));
   AHIP(a, qg::launch_adj_col_out(d, a->st, b));
   return QG_OK;
@@ -508,14 +524,26 @@ int load_state(qg_adj* a, const double2* src, const int* meta) {
   return QG_OK;
 }
 
-// keep psi, q, BR, BI of the member in segment slot j
+// keep psi, q, BR, BI of the member in segment slot j: whole fields before
+// the Euler step (the initial B is not yet box-restricted), the dealiasing
+// box afterwards (the only part the adjoint step reads)
 int keep_step(qg_adj* a, int j) {
   qg_ctx* c = a->c;
   AHIP(a, hipStreamWaitEvent(c->stream, c->evA, 0));
   const double2* f[4] = {c->d.psi, c->d.q, c->d.BR, c->d.BI};
-  for (int i = 0; i < 4; ++i)
-    AHIP(a, hipMemcpyAsync(a->seg + ((size_t)j * 4 + i) * a->S, f[i], a->S * sizeof(double2),
-                           hipMemcpyDeviceToDevice, a->st));
+  if (c->iter == 0) {
+    for (int i = 0; i < 4; ++i)
+      AHIP(a, hipMemcpyAsync(a->seg + ((size_t)j * 4 + i) * a->S, f[i], a->S * sizeof(double2),
+                             hipMemcpyDeviceToDevice, a->st));
+  } else {
+    qg::CopyBoxArgs cb;
+    for (int i = 0; i < 4; ++i) {
+      cb.src[i] = f[i];
+      cb.dst[i] = a->seg + ((size_t)j * 4 + i) * a->S;
+    }
+    cb.n = 4;
+    AHIP(a, qg::launch_copy_box(c->d.g, a->st, cb));
+  }
   return QG_OK;
 }
 const double2* seg_field(const qg_adj* a, int j, int i) { return a->seg + ((size_t)j * 4 + i) * a->S; }
@@ -540,9 +568,18 @@ int forward(qg_adj* a, const double* psir, const double* BRr, const double* BIr,
     a->ckpt.push_back(b);
   }
   a->ckpt_meta.assign(3 * (size_t)nck, 0);
+  // the last segment's states are kept as the forward run passes them, so
+  // the adjoint sweep does not recompute that segment
+  const int n_last = checkpoints ? (nck - 1) * a->K : nsteps;
+  a->last_seg = -1;
   for (int n = 0; n < nsteps; ++n) {
     if (checkpoints && n % a->K == 0) ARC(save_state(a, a->ckpt[n / a->K], &a->ckpt_meta[3 * (n / a->K)]));
+    if (n >= n_last) ARC(keep_step(a, n - n_last));
     ARC(fstep(a));
+  }
+  if (checkpoints) {
+    ARC(keep_step(a, nsteps - n_last));
+    a->last_seg = nck - 1;
   }
   AHIP(a, hipStreamWaitEvent(c->stream, c->evA, 0));
   int flag = 0;
@@ -572,12 +609,15 @@ int sweep(qg_adj* a, int nsteps) {
   for (int cidx = nck - 1; cidx >= 0; --cidx) {
     const int n0 = cidx * a->K;
     const int len = std::min(a->K, nsteps - n0);
-    ARC(load_state(a, a->ckpt[cidx], &a->ckpt_meta[3 * cidx]));
-    ARC(keep_step(a, 0));
-    for (int j = 1; j <= len; ++j) {
-      ARC(fstep(a));
-      ARC(keep_step(a, j));
+    if (cidx != a->last_seg) {
+      ARC(load_state(a, a->ckpt[cidx], &a->ckpt_meta[3 * cidx]));
+      ARC(keep_step(a, 0));
+      for (int j = 1; j <= len; ++j) {
+        ARC(fstep(a));
+        ARC(keep_step(a, j));
+      }
     }
+    a->last_seg = -1;
     for (int j = len - 1; j >= 0; --j) {
       const double2* fn[4] = {seg_field(a, j, 0), seg_field(a, j, 1), seg_field(a, j, 2), seg_field(a, j, 3)};
       if (a->fused)
@@ -597,8 +637,11 @@ int control_adjoint(qg_adj* a, double* grad_host, int ldx) {
   AHIP(a, qg::launch_init_q(dd, a->st, a->x[F_Q], a->s[9]));
   ARC(o.c2r_of(a->p[12], {wpow(term(a->x[F_PSI], 1.0 / o.N()), -1), wpow(term(a->s[9], 1.0 / o.N()), -1)}));
   const auto& g = a->w.g;
-  AHIP(a, hipMemcpy2DAsync(grad_host, (size_t)ldx * sizeof(double), a->p[12], (size_t)g.n1 * sizeof(double),
-                           (size_t)g.n1 * sizeof(double), (size_t)g.n2 * g.nz, hipMemcpyDeviceToHost, a->st));
+  if (ldx == g.n1)
+    AHIP(a, hipMemcpyAsync(grad_host, a->p[12], a->NP * sizeof(double), hipMemcpyDeviceToHost, a->st));
+  else
+    AHIP(a, hipMemcpy2DAsync(grad_host, (size_t)ldx * sizeof(double), a->p[12], (size_t)g.n1 * sizeof(double),
+                             (size_t)g.n1 * sizeof(double), (size_t)g.n2 * g.nz, hipMemcpyDeviceToHost, a->st));
   AHIP(a, hipStreamSynchronize(a->st));
   return QG_OK;
 }
@@ -614,9 +657,7 @@ int check(qg_adj* a, const double* psir, const double* BRr, const double* BIr, i
 int h2d_state(qg_adj* a, const double* host) {
   const auto& g = a->w.g;
   for (int f = 0; f < NSTATE; ++f)
-    AHIP(a, hipMemcpy2DAsync(a->x[f], (size_t)g.sp * sizeof(double2),
-                             host + (size_t)f * g.iktx * g.ikty * g.nz * 2, (size_t)g.iktx * sizeof(double2),
-                             (size_t)g.iktx * sizeof(double2), (size_t)g.ikty * g.nz, hipMemcpyHostToDevice, a->st));
+    AHIP(a, qgi::h2d_field(g, a->x[f], host + (size_t)f * g.iktx * g.ikty * g.nz * 2, g.nz, a->st, a->c->stage));
   // keep only L = 1 modes
   Ops o{a};
   for (int f = 0; f < NSTATE; ++f) ARC(o.smul(a->x[f], {term(a->x[f])}));
@@ -626,10 +667,7 @@ int h2d_state(qg_adj* a, const double* host) {
 int d2h_state(qg_adj* a, double* host) {
   const auto& g = a->w.g;
   for (int f = 0; f < NSTATE; ++f)
-    AHIP(a, hipMemcpy2DAsync(host + (size_t)f * g.iktx * g.ikty * g.nz * 2, (size_t)g.iktx * sizeof(double2), a->x[f],
-                             (size_t)g.sp * sizeof(double2), (size_t)g.iktx * sizeof(double2), (size_t)g.ikty * g.nz,
-                             hipMemcpyDeviceToHost, a->st));
-  AHIP(a, hipStreamSynchronize(a->st));
+    AHIP(a, qgi::d2h_field(g, host + (size_t)f * g.iktx * g.ikty * g.nz * 2, a->x[f], g.nz, a->st, a->c->stage));
   return QG_OK;
 }
 
@@ -661,7 +699,14 @@ int qg_adj_create(qg_ctx* c, int32_t checkpoint_every, double z_deep, qg_adj** o
   for (int i = 0; i < NSTATE && rc == QG_OK; ++i) rc = aalloc(a, &a->x[i], a->S);
   for (int i = 0; i < 12 && rc == QG_OK; ++i) rc = aalloc(a, &a->s[i], a->S);
   for (int i = 0; i < 24 && rc == QG_OK; ++i) rc = aalloc(a, &a->p[i], a->NP);
-  for (int i = 0; i < 6 && rc == QG_OK; ++i) rc = aalloc(a, &a->F[i], a->S);
+  // forward planes in the order F0, F2, F4, F1, F3, F5 so that the y-inverses
+  // of (q, BR, BI) and of (BR', BI') run as single launches over stacked levels
+  double2* fblk = nullptr;
+  if (rc == QG_OK) rc = aalloc(a, &fblk, 6 * a->S);
+  if (rc == QG_OK) {
+    const int order[6] = {0, 3, 1, 4, 2, 5};
+    for (int i = 0; i < 6; ++i) a->F[i] = fblk + (size_t)order[i] * a->S;
+  }
   for (int i = 0; i < 6 && rc == QG_OK; ++i) rc = aalloc(a, &a->A[i], a->S);
   double2* tq = nullptr;
   if (rc == QG_OK) rc = aalloc(a, &tq, a->S);
@@ -716,8 +761,11 @@ int qg_adj_tlm(qg_adj* a, const double* psir, const double* BRr, const double* B
   // perturbation of the initial state: psi-hat = r2c(dpsi), q = init_q(psi-hat)
   const auto& g = a->w.g;
   for (int f = 0; f < NSTATE; ++f) ARC(o.zero(a->x[f]));
-  AHIP(a, hipMemcpy2DAsync(a->p[12], (size_t)g.n1 * sizeof(double), dpsir, (size_t)ldx * sizeof(double),
-                           (size_t)g.n1 * sizeof(double), (size_t)g.n2 * g.nz, hipMemcpyHostToDevice, a->st));
+  if (ldx == g.n1)
+    AHIP(a, hipMemcpyAsync(a->p[12], dpsir, a->NP * sizeof(double), hipMemcpyHostToDevice, a->st));
+  else
+    AHIP(a, hipMemcpy2DAsync(a->p[12], (size_t)g.n1 * sizeof(double), dpsir, (size_t)ldx * sizeof(double),
+                             (size_t)g.n1 * sizeof(double), (size_t)g.n2 * g.nz, hipMemcpyHostToDevice, a->st));
   ARC(o.r2c(a->p[12], a->s[9]));
   // only the L = 1 modes of psi-hat act in the step (every use is masked);
   // the inversions write the box only, so the rest must stay zero

@@ -53,6 +53,13 @@ hipError_t launch_adj_col_upd(const Dev& d, hipStream_t st, const AdjBufs& b, in
 hipError_t launch_adj_row_nl(const Dev& d, hipStream_t st, const AdjBufs& b, int Win, int euler);
 hipError_t launch_adj_col_out(const Dev& d, hipStream_t st, const AdjBufs& b);
 
+struct CopyBoxArgs {
+  const double2* src[4];
+  double2* dst[4];
+  int n;
+};
+hipError_t launch_copy_box(const Geo& g, hipStream_t st, const CopyBoxArgs& a);
+
 hipError_t launch_smul(const Dev& d, hipStream_t st, double2* out, const SMulArgs& a);
 hipError_t launch_pfma(hipStream_t st, double* out, size_t n, const PFmaArgs& a);
 int obj_blocks(const Geo& g);

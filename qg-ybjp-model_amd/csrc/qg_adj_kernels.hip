@@ -10,6 +10,7 @@
 //    (2, and 1 at kx = ky = 0)
 //  - k_pfma: physical multiply-add of up to four products
 //  - k_obj: partial sums of sum W kh2 (|AR|^2 + |AI|^2) over a level range
+//  - k_copy_box: copy of the dealiasing box of up to four fields
 #include <hip/hip_runtime.h>
 
 #include "qg_adj.hpp"
@@ -115,7 +116,26 @@ __global__ __launch_bounds__(NTA) void k_obj(Dev d, const double2* __restrict__ 
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
+// copy the dealiasing box of up to four fields (every member level)
+__global__ __launch_bounds__(NTA) void k_copy_box(Geo g, CopyBoxArgs a) {
+  const size_t per = (size_t)g.KX * g.nbox;
+  const size_t e = (size_t)blockIdx.x * NTA + threadIdx.x;
+  if (e >= per * g.nz) return;
+  const size_t z = e / per, r = e % per;
+  const int kx = (int)(r % g.KX), iky = box_row((int)(r / g.KX), g);
+  const size_t idx = z * g.ikty * g.sp + (size_t)iky * g.sp + kx;
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+    if (j < a.n) a.dst[j][idx] = a.src[j][idx];
+}
+
 }  // namespace
+
+hipError_t launch_copy_box(const Geo& g, hipStream_t st, const CopyBoxArgs& a) {
+  const size_t n = (size_t)g.KX * g.nbox * g.nz;
+  hipLaunchKernelGGL(k_copy_box, dim3((unsigned)((n + NTA - 1) / NTA)), dim3(NTA), 0, st, g, a);
+  return hipGetLastError();
+}
 
 hipError_t launch_smul(const Dev& d, hipStream_t st, double2* out, const SMulArgs& a) {
   const size_t n = (size_t)d.g.ikty * d.g.sp * d.g.nz;

@@ -78,14 +78,21 @@ int dalloc(qg_ctx* c, T** p, size_t n) {
 
 size_t field_elems(const Geo& g) { return (size_t)g.nz * g.ikty * g.sp; }
 
-// host (iktx, ikty, nz) complex <-> device rows of pitch sp
-hipError_t h2d_field(const Geo& g, double2* dst, const double* src, int nz, hipStream_t st) {
-  return hipMemcpy2DAsync(dst, (size_t)g.sp * sizeof(double2), src, (size_t)g.iktx * sizeof(double2),
-                          (size_t)g.iktx * sizeof(double2), (size_t)g.ikty * nz, hipMemcpyHostToDevice, st);
+// host (iktx, ikty, nz) complex <-> device rows of pitch sp, through a
+// contiguous device staging buffer (one host transfer, the re-pitching as a
+// device-side 2D copy)
+hipError_t h2d_field(const Geo& g, double2* dst, const double* src, int nz, hipStream_t st, double2* stage) {
+  const size_t n = (size_t)g.iktx * g.ikty * nz;
+  hipError_t e = hipMemcpyAsync(stage, src, n * sizeof(double2), hipMemcpyHostToDevice, st);
+  if (e != hipSuccess) return e;
+  return hipMemcpy2DAsync(dst, (size_t)g.sp * sizeof(double2), stage, (size_t)g.iktx * sizeof(double2),
+                          (size_t)g.iktx * sizeof(double2), (size_t)g.ikty * nz, hipMemcpyDeviceToDevice, st);
 }
-hipError_t d2h_field(const Geo& g, double* dst, const double2* src, int nz, hipStream_t st) {
-  hipError_t e = hipMemcpy2DAsync(dst, (size_t)g.iktx * sizeof(double2), src, (size_t)g.sp * sizeof(double2),
-                                  (size_t)g.iktx * sizeof(double2), (size_t)g.ikty * nz, hipMemcpyDeviceToHost, st);
+hipError_t d2h_field(const Geo& g, double* dst, const double2* src, int nz, hipStream_t st, double2* stage) {
+  const size_t n = (size_t)g.iktx * g.ikty * nz;
+  hipError_t e = hipMemcpy2DAsync(stage, (size_t)g.iktx * sizeof(double2), src, (size_t)g.sp * sizeof(double2),
+                                  (size_t)g.iktx * sizeof(double2), (size_t)g.ikty * nz, hipMemcpyDeviceToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(dst, stage, n * sizeof(double2), hipMemcpyDeviceToHost, st);
   if (e != hipSuccess) return e;
   return hipStreamSynchronize(st);
 }
@@ -328,7 +335,7 @@ int qg_create_batch(const qg_params* p, int32_t members, qg_ctx** out) {
   d.dz = (4.0 * std::asin(1.0)) / p->n3;  // dz = L3/n3 (parameters_test1.f90:14)
   const size_t S = field_elems(g);
   const size_t plane = (size_t)g.ikty * g.iktx;
-  double2** fields[] = {&d.psi, &d.q, &d.qo, &d.BR, &d.BI, &d.BRo, &d.BIo, &d.AR, &d.AI, &d.qs, &d.dq};
+  double2** fields[] = {&d.psi, &d.q, &d.qo, &d.BR, &d.BI, &d.BRo, &d.BIo, &d.AR, &d.AI, &d.qs, &d.dq, &c->stage};
   for (auto f : fields)
     if ((rc = dalloc(c, f, S)) != QG_OK) { qg_destroy(c); return rc; }
   for (int i = 0; i < 6; ++i)
@@ -414,7 +421,7 @@ int qg_set_state(qg_ctx* c, const double* psik, const double* qk, const double* 
   double2* dst[] = {d.psi, d.q, d.qo, d.BR, d.BI, d.BRo, d.BIo, d.AR, d.AI};
   for (int i = 0; i < 9; ++i) {
     HIPCHK(c, qg::launch_clear_outside(d, c->stream, dst[i], 1));
-    if (src[i]) HIPCHK(c, h2d_field(d.g, dst[i], src[i], d.g.nz, c->stream));
+    if (src[i]) HIPCHK(c, h2d_field(d.g, dst[i], src[i], d.g.nz, c->stream, c->stage));
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->iter = iter;
@@ -463,14 +470,14 @@ int qg_get_state(qg_ctx* c, double* psik, double* qk, double* qok, double* BRk, 
   double* dst[] = {psik, qk, qok, BRk, BIk, BRok, BIok, ARk, AIk};
   const double2* src[] = {d.psi, d.q, d.qo, d.BR, d.BI, d.BRo, d.BIo, d.AR, d.AI};
   for (int i = 0; i < 9; ++i)
-    if (dst[i]) HIPCHK(c, d2h_field(d.g, dst[i], src[i], d.g.nz, c->stream));
+    if (dst[i]) HIPCHK(c, d2h_field(d.g, dst[i], src[i], d.g.nz, c->stream, c->stage));
   if (CRk) {
     HIPCHK(c, qg::launch_cz(d, c->stream, d.AR, d.qs));
-    HIPCHK(c, d2h_field(d.g, CRk, d.qs, d.g.nz, c->stream));
+    HIPCHK(c, d2h_field(d.g, CRk, d.qs, d.g.nz, c->stream, c->stage));
   }
   if (CIk) {
     HIPCHK(c, qg::launch_cz(d, c->stream, d.AI, d.qs));
-    HIPCHK(c, d2h_field(d.g, CIk, d.qs, d.g.nz, c->stream));
+    HIPCHK(c, d2h_field(d.g, CIk, d.qs, d.g.nz, c->stream, c->stage));
   }
   return QG_OK;
 }
@@ -523,9 +530,9 @@ int qg_init_q(qg_ctx* c, const double* psik, double* qk) {
   if (!psik || !qk) return fail(c, QG_ERR_ARG, "null array");
   Dev& d = c->d;
   HIPCHK(c, qg::launch_clear_outside(d, c->stream, d.qs, 1));
-  HIPCHK(c, h2d_field(d.g, d.qs, psik, d.g.nz, c->stream));
+  HIPCHK(c, h2d_field(d.g, d.qs, psik, d.g.nz, c->stream, c->stage));
   HIPCHK(c, qg::launch_init_q(d, c->stream, d.qs, d.dq));
-  HIPCHK(c, d2h_field(d.g, qk, d.dq, d.g.nz, c->stream));
+  HIPCHK(c, d2h_field(d.g, qk, d.dq, d.g.nz, c->stream, c->stage));
   return QG_OK;
 }
 
@@ -540,7 +547,7 @@ int qg_fft_r2c(qg_ctx* c, const double* r, int32_t ldx, double* k, int32_t nz) {
   HIPCHK(c, hipMalloc(&tmp, nr * sizeof(double)));
   hipError_t e = hipMemcpyAsync(tmp, r, nr * sizeof(double), hipMemcpyHostToDevice, c->stream);
   if (e == hipSuccess) e = qg::launch_r2c(d, c->stream, tmp, ldx, d.dq, nz);
-  if (e == hipSuccess) e = d2h_field(g, k, d.dq, nz, c->stream);
+  if (e == hipSuccess) e = d2h_field(g, k, d.dq, nz, c->stream, c->stage);
   hipFree(tmp);
   if (e != hipSuccess) return fail_hip(c, e, "qg_fft_r2c");
   c->p_valid = 0;
@@ -557,7 +564,7 @@ int qg_fft_c2r(qg_ctx* c, const double* k, double* r, int32_t ldx, int32_t nz) {
   double* tmp = nullptr;
   HIPCHK(c, hipMalloc(&tmp, nr * sizeof(double)));
   hipError_t e = hipMemsetAsync(tmp, 0, nr * sizeof(double), c->stream);
-  if (e == hipSuccess) e = h2d_field(g, d.dq, k, nz, c->stream);
+  if (e == hipSuccess) e = h2d_field(g, d.dq, k, nz, c->stream, c->stage);
   if (e == hipSuccess) e = qg::launch_c2r(d, c->stream, d.dq, tmp, ldx, nz);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e == hipSuccess) e = hipMemcpy(r, tmp, nr * sizeof(double), hipMemcpyDeviceToHost);

@@ -36,6 +36,7 @@ struct qg_ctx {
   int capturing = 0;
   int evA_live = 1;
   int generation = 0;  // bumped whenever captured launches would change
+  double2* stage = nullptr;  // contiguous staging for host transfers (one field)
 };
 
 namespace qgi {
@@ -47,6 +48,10 @@ int step_once(qg_ctx* c);
 // make the context's device current
 int check_ctx(qg_ctx* c);
 size_t field_elems(const qg::Geo& g);
+// host (iktx, ikty, nz) complex arrays <-> pitched device fields via `stage`
+// (d2h synchronises the stream)
+hipError_t h2d_field(const qg::Geo& g, double2* dst, const double* src, int nz, hipStream_t st, double2* stage);
+hipError_t d2h_field(const qg::Geo& g, double* dst, const double2* src, int nz, hipStream_t st, double2* stage);
 // true once the step sequence no longer changes from step to step (after the
 // Euler step, the first full-width transforms and the clearing steps)
 inline bool steady(const qg_ctx* c) { return c->iter >= 1 && c->p_valid && c->clear_pending == 0; }
