@@ -158,9 +158,11 @@ const char* qg_kernel_name(int32_t id);
 
 /* Kernel variants, v[0..n-1] with n <= QG_NVARIANTS:
  *  v[0..4] the five transform kernels (ps_inv, row_nl, col_upd, row_qw,
- *          col_qw).  Column kernels: 0 = 8 elements per thread, 1 = 16.  Row
- *          kernels: 0 = 8 per thread in 256-thread blocks, 1 = 16/256,
- *          2 = 8/64, 3 = 16/64.
+ *          col_qw).  ps_inv: 0 = 8 elements per thread, 1 = 16.  col_upd,
+ *          col_qw: 0 = 8, 1 = 16 with two LDS exchange buffers per column,
+ *          2 = 8, 3 = 16 with one (half the LDS, more barriers).  Row
+ *          kernels (row_nl, row_qw): 0 = 8 elements per thread in 64-thread
+ *          blocks, 1 = 16/64, 2 = 8/128, 3 = 16/32.
  *  v[5]    shape of the partitioned vertical solve: 0..3 = (levels per
  *          chunk, columns per block) of (8,16), (8,32), (16,16), (16,32).
  *  v[6]    1: the A inversions run in the psi inversion's launch on the
@@ -198,10 +200,11 @@ int qg_ensemble_step(qg_ens* e, int32_t nsteps);
 int qg_tune(qg_ctx* ctx, int32_t steps, int32_t* best, double* best_ms);
 
 /* Tangent-linear and adjoint of the step, for gradients with respect to the
- * initial streamfunction.  An adjoint context is bound to one single-member
- * context with nuz = 0 and flags 0 or QG_NO_FEEDBACK, whose state it uses as
- * the forward trajectory (the member is left at the final state of the last
- * run).  Objective: J = 1/2 * (2 pi)^2 dz * sum over the levels with
+ * initial streamfunction.  An adjoint context is bound to a context (one
+ * member, or a batch whose members get independent gradients in the same
+ * launches) with nuz = 0 and flags 0 or QG_NO_FEEDBACK, whose state it uses
+ * as the forward trajectory (left at the final state of the last run).  J
+ * and dJ return one value per member.  Objective: J = 1/2 * (2 pi)^2 dz * sum over the levels with
  * z_s < z_deep of sum over L = 1 modes of W kh2 (|AR|^2 + |AI|^2) after
  * nsteps, i.e. the domain integral of |grad_h A|^2 / 2 below the depth
  * L3 - z_deep (W: 2 per half-spectrum mode, 1 at kx = ky = 0).  Controls and

@@ -23,7 +23,8 @@
 // forward kernels (storing psi, q, BR, BI of every step) and runs the adjoint
 // steps of that segment backwards.
 //
-// Scope: one member (members == 1), nuz = 0, flags 0 or QG_NO_FEEDBACK.
+// Scope: nuz = 0, flags 0 or QG_NO_FEEDBACK; a batch context gives the
+// members' independent gradients (and objectives) in one pass.
 // No counterpart in the reference.
 #include <hip/hip_runtime.h>
 
@@ -589,22 +590,40 @@ int forward(qg_adj* a, const double* psir, const double* BRr, const double* BIr,
   return QG_OK;
 }
 
+// J of every member: 1/2 (2 pi)^2 dz sum over the deep levels of W kh2 (|AR|^2 + |AI|^2)
 int objective(qg_adj* a, const double2* AR, const double2* AI, double* J) {
-  const int nb = qg::obj_blocks(a->w.g);
-  AHIP(a, qg::launch_obj(a->w, a->st, AR, AI, 0, a->zhi, a->part));
-  std::vector<double> h(nb);
-  AHIP(a, hipMemcpyAsync(h.data(), a->part, nb * sizeof(double), hipMemcpyDeviceToHost, a->st));
+  const auto& g = a->w.g;
+  Dev dm = a->w;  // one member's view
+  dm.g.nm = 1;
+  dm.g.nz = g.n3;
+  const int nb = qg::obj_blocks(dm.g);
+  const size_t moff = (size_t)g.n3 * g.ikty * g.sp;
+  for (int m = 0; m < g.nm; ++m)
+    AHIP(a, qg::launch_obj(dm, a->st, AR + m * moff, AI + m * moff, 0, a->zhi, a->part + (size_t)m * nb));
+  std::vector<double> h((size_t)nb * g.nm);
+  AHIP(a, hipMemcpyAsync(h.data(), a->part, h.size() * sizeof(double), hipMemcpyDeviceToHost, a->st));
   AHIP(a, hipStreamSynchronize(a->st));
-  double s = 0.0;
-  for (double v : h) s += v;
   const double twopi = 4.0 * std::asin(1.0);
-  *J = 0.5 * twopi * twopi * a->c->d.dz * s;
+  for (int m = 0; m < g.nm; ++m) {
+    double sum = 0.0;
+    for (int b = 0; b < nb; ++b) sum += h[(size_t)m * nb + b];
+    J[m] = 0.5 * twopi * twopi * a->c->d.dz * sum;
+  }
+  return QG_OK;
+}
+
+// physical work fields of the generic (unfused) linearised step, allocated
+// on first use
+int ensure_generic(qg_adj* a) {
+  for (int i = 0; i < 24; ++i)
+    if (!a->p[i]) ARC(aalloc(a, &a->p[i], a->NP));
   return QG_OK;
 }
 
 // adjoint sweep over nsteps given x = adjoint of the final state; leaves the
 // adjoint of the initial (psi-hat, q) in x[F_PSI], x[F_Q]
 int sweep(qg_adj* a, int nsteps) {
+  if (!a->fused) ARC(ensure_generic(a));
   const int nck = (nsteps + a->K - 1) / a->K;
   for (int cidx = nck - 1; cidx >= 0; --cidx) {
     const int n0 = cidx * a->K;
@@ -678,7 +697,6 @@ extern "C" {
 int qg_adj_create(qg_ctx* c, int32_t checkpoint_every, double z_deep, qg_adj** out) {
   if (!c || !out || checkpoint_every < 1) return QG_ERR_ARG;
   *out = nullptr;
-  if (c->d.g.nm != 1) return qgi::fail(c, QG_ERR_ARG, "qg_adj_create: one member per context");
   if (c->d.nuz != 0.0 || (c->d.flags & ~qg::F_NO_FEEDBACK))
     return qgi::fail(c, QG_ERR_ARG, "qg_adj_create: needs nuz = 0 and flags 0 or QG_NO_FEEDBACK");
   int rc = qgi::check_ctx(c);
@@ -698,7 +716,7 @@ int qg_adj_create(qg_ctx* c, int32_t checkpoint_every, double z_deep, qg_adj** o
   rc = QG_OK;
   for (int i = 0; i < NSTATE && rc == QG_OK; ++i) rc = aalloc(a, &a->x[i], a->S);
   for (int i = 0; i < 12 && rc == QG_OK; ++i) rc = aalloc(a, &a->s[i], a->S);
-  for (int i = 0; i < 24 && rc == QG_OK; ++i) rc = aalloc(a, &a->p[i], a->NP);
+  rc = aalloc(a, &a->p[12], a->NP);  // control transfers; the rest on first use
   // forward planes in the order F0, F2, F4, F1, F3, F5 so that the y-inverses
   // of (q, BR, BI) and of (BR', BI') run as single launches over stacked levels
   double2* fblk = nullptr;
@@ -711,7 +729,12 @@ int qg_adj_create(qg_ctx* c, int32_t checkpoint_every, double z_deep, qg_adj** o
   double2* tq = nullptr;
   if (rc == QG_OK) rc = aalloc(a, &tq, a->S);
   a->w.qs = tq;  // transform temporary of the raw r2c / c2r
-  if (rc == QG_OK) rc = aalloc(a, &a->part, (size_t)qg::obj_blocks(g));
+  if (rc == QG_OK) {
+    qg::Geo gm = g;
+    gm.nm = 1;
+    gm.nz = g.n3;
+    rc = aalloc(a, &a->part, (size_t)qg::obj_blocks(gm) * g.nm);
+  }
   if (rc == QG_OK) rc = aalloc(a, &a->seg, (size_t)(a->K + 1) * 4 * a->S);
   if (rc == QG_OK) rc = aalloc(a, &a->fin, 12 * a->S);
   if (rc == QG_OK && hipStreamSynchronize(a->st) != hipSuccess) rc = QG_ERR_HIP;
@@ -753,6 +776,7 @@ int qg_adj_tlm(qg_adj* a, const double* psir, const double* BRr, const double* B
                const double* dpsir, double* dJ, double* dstate) {
   ARC(check(a, psir, BRr, BIr, ldx, nsteps));
   if (!dpsir) return afail(a, QG_ERR_ARG, "null perturbation");
+  ARC(ensure_generic(a));
   qg_ctx* c = a->c;
   Ops o{a};
   int rc = qg_set_state_physical(c, psir, BRr, BIr, ldx);
@@ -785,10 +809,11 @@ int qg_adj_tlm(qg_adj* a, const double* psir, const double* BRr, const double* B
     ARC(o.smul(a->s[11], {term(c->d.AI), term(a->x[F_AI])}));
     ARC(o.smul(a->s[8], {term(c->d.AI), term(a->x[F_AI], -1.0)}));
     // |a+b|^2 - |a-b|^2 = 4 Re(conj(a) b)
-    double j1 = 0, j2 = 0;
-    ARC(objective(a, a->s[9], a->s[11], &j1));
-    ARC(objective(a, a->s[10], a->s[8], &j2));
-    *dJ = 2.0 * (j1 - j2) / 4.0;
+    const int nm = a->w.g.nm;
+    std::vector<double> j1(nm), j2(nm);
+    ARC(objective(a, a->s[9], a->s[11], j1.data()));
+    ARC(objective(a, a->s[10], a->s[8], j2.data()));
+    for (int m = 0; m < nm; ++m) dJ[m] = 2.0 * (j1[m] - j2[m]) / 4.0;
   }
   if (dstate) ARC(d2h_state(a, dstate));
   AHIP(a, hipStreamSynchronize(a->st));
@@ -814,9 +839,10 @@ int qg_adj_gradient(qg_adj* a, const double* psir, const double* BRr, const doub
   ARC(check(a, psir, BRr, BIr, ldx, nsteps));
   if (!grad) return afail(a, QG_ERR_ARG, "null gradient");
   ARC(forward(a, psir, BRr, BIr, ldx, nsteps, true));
-  double Jv = 0.0;
-  ARC(objective(a, a->c->d.AR, a->c->d.AI, &Jv));
-  if (J) *J = Jv;
+  std::vector<double> Jv(a->w.g.nm);
+  ARC(objective(a, a->c->d.AR, a->c->d.AI, Jv.data()));
+  if (J)
+    for (int m = 0; m < a->w.g.nm; ++m) J[m] = Jv[m];
   ARC(save_state(a, a->fin, a->fin_meta));
   // dJ/dA at the final step: (2 pi)^2 dz W kh2 A on the deep levels
   Ops o{a};

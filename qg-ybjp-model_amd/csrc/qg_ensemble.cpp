@@ -273,9 +273,9 @@ int qg_tune(qg_ctx* c, int32_t steps, int32_t* best, double* best_ms) {
     return QG_OK;
   };
 
-  // variant ranges: column kernels (0, 2, 4) have 2 shapes, row kernels 4,
-  // the solve 4, the A placement 2
-  const int hi[QG_NVARIANTS] = {1, 3, 1, 3, 1, 3, 1};
+  // variant ranges: ps_inv 2 shapes, the other transform kernels 4, the
+  // solve 4, the A placement 2
+  const int hi[QG_NVARIANTS] = {1, 3, 3, 3, 3, 3, 1};
   qg::Tune cur = d.tune;
   auto get = [&](const qg::Tune& t, int i) { return i < 5 ? t.e16[i] : (i == 5 ? t.solve : t.a_main); };
   auto set = [&](qg::Tune& t, int i, int v) {

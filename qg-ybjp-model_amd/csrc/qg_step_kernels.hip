@@ -189,14 +189,14 @@ __global__ __launch_bounds__(RP*(N / E)) void k_row_nl(Dev d, int Win, double sc
 // the new field for the next step's products and for the feedback term
 // (derivatives.f90:938-956).  blockIdx.z: 0 = q, 1 = BR, 2 = BI.
 // ---------------------------------------------------------------------------
-template <int N, int E>
+template <int N, int E, int SB>
 __global__ __launch_bounds__((SH<N, E>::COLT)) void k_col_upd(Dev d, int euler, int has_dq) {
   constexpr int T = SH<N, E>::T, TX = SH<N, E>::TX, XL = xbuf_len<N, E>();
-  __shared__ double2 lds[TX * 2 * XL];
+  __shared__ double2 lds[TX * SB * XL];
   const Geo& g = d.g;
   const int c = threadIdx.x % TX, t = threadIdx.x / TX;
   const int kx0 = blockIdx.x * TX, kx = kx0 + c, z = blockIdx.y, f = blockIdx.z;
-  double2* xb = lds + c * 2 * XL;
+  double2* xb = lds + c * SB * XL;
   double2* W1 = d.I[2 * f];
   double2* W2 = d.I[2 * f + 1];
   const size_t pz = (size_t)z * g.n2 * g.sp;
@@ -209,7 +209,7 @@ __global__ __launch_bounds__((SH<N, E>::COLT)) void k_col_upd(Dev d, int euler, 
     v[0][m] = colok ? (W1 + pz + m * mstride)[toff] : zero2();
     v[1][m] = colok ? (W2 + pz + m * mstride)[toff] : zero2();
   }
-  reg_fft_multi<N, E, 2, 2, false>(v, xb, t, d.tw);
+  reg_fft_multi<N, E, 2, SB, false>(v, xb, t, d.tw);
 
   double2* X = f == 0 ? d.q : (f == 1 ? d.BR : d.BI);
   double2* Xo = f == 0 ? d.qo : (f == 1 ? d.BRo : d.BIo);
@@ -277,19 +277,38 @@ __global__ __launch_bounds__((SH<N, E>::COLT)) void k_col_upd(Dev d, int euler, 
   if (bad) atomicOr(d.flag, 1);
   if (kx0 == 0) {
     // realit on the kx = 0 column of the new field and of i ky times it
-    rf_publish_range<N, E, 2, 0, 2>(v, xb, t);
-    if (c == 0) {
+    if constexpr (SB == 2) {
+      rf_publish_range<N, E, 2, 0, 2>(v, xb, t);
+      if (c == 0) {
 #pragma unroll
-      for (int m = 0; m < E; ++m) {
-        const int iky = t + T * m;
-        if (iky > N / 2) {
-          v[0][m] = cconj(xb[xpad<E>(N - iky)]);
-          v[1][m] = cconj(xb[XL + xpad<E>(N - iky)]);
+        for (int m = 0; m < E; ++m) {
+          const int iky = t + T * m;
+          if (iky > N / 2) {
+            v[0][m] = cconj(xb[xpad<E>(N - iky)]);
+            v[1][m] = cconj(xb[XL + xpad<E>(N - iky)]);
+          }
+        }
+      }
+    } else {
+      rf_publish_range<N, E, 2, 0, 1>(v, xb, t);
+      if (c == 0) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+          const int iky = t + T * m;
+          if (iky > N / 2) v[0][m] = cconj(xb[xpad<E>(N - iky)]);
+        }
+      }
+      rf_publish_range<N, E, 2, 1, 1>(v, xb, t);
+      if (c == 0) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+          const int iky = t + T * m;
+          if (iky > N / 2) v[1][m] = cconj(xb[xpad<E>(N - iky)]);
         }
       }
     }
   }
-  reg_fft_multi<N, E, 2, 2, true>(v, xb, t, d.tw);
+  reg_fft_multi<N, E, 2, SB, true>(v, xb, t, d.tw);
   if (colok) {
 #pragma unroll
     for (int m = 0; m < E; ++m) (W1 + pz + m * mstride)[toff] = v[0][m];
@@ -356,14 +375,14 @@ __global__ __launch_bounds__(RP*(N / E)) void k_row_qw(Dev d, double scale) {
 // (derivatives.f90:1010-1027) and q* = q - q^w where L = 1
 // (main_waqg.f90:430-439)
 // ---------------------------------------------------------------------------
-template <int N, int E>
+template <int N, int E, int SB>
 __global__ __launch_bounds__((SH<N, E>::COLT)) void k_col_qw(Dev d) {
   constexpr int T = SH<N, E>::T, TX = SH<N, E>::TX, XL = xbuf_len<N, E>();
-  __shared__ double2 lds[TX * 2 * XL];
+  __shared__ double2 lds[TX * SB * XL];
   const Geo& g = d.g;
   const int c = threadIdx.x % TX, t = threadIdx.x / TX;
   const int kx0 = blockIdx.x * TX, kx = kx0 + c, z = blockIdx.y;
-  double2* xb = lds + c * 2 * XL;
+  double2* xb = lds + c * SB * XL;
   const size_t pz = (size_t)z * g.n2 * g.sp;
   const bool colok = kx < g.KX;
   const int toff = t * g.sp + kx;
@@ -374,7 +393,7 @@ __global__ __launch_bounds__((SH<N, E>::COLT)) void k_col_qw(Dev d) {
     v[0][m] = colok ? (d.I[3] + pz + m * mstride)[toff] : zero2();
     v[1][m] = colok ? (d.I[5] + pz + m * mstride)[toff] : zero2();
   }
-  reg_fft_multi<N, E, 2, 2, false>(v, xb, t, d.tw);
+  reg_fft_multi<N, E, 2, SB, false>(v, xb, t, d.tw);
   if (!colok) return;
   const size_t sz = (size_t)z * g.ikty * g.sp;
 #pragma unroll
@@ -414,11 +433,11 @@ struct LS {
     hipLaunchKernelGGL((k_row_nl<N, E, RP, 2>), dim3(d.g.n2 * d.g.nz / RP), dim3(RP * (N / E)), 0, st, d, Win, scale);
     return hipGetLastError();
   }
-  template <int E>
+  template <int E, int SB>
   static hipError_t col_upd_e(const Dev& d, hipStream_t st, int euler, int has_dq) {
     using S = SH<N, E>;
     dim3 grid((d.g.KX + S::TX - 1) / S::TX, d.g.nz, 3);
-    hipLaunchKernelGGL((k_col_upd<N, E>), grid, dim3(S::COLT), 0, st, d, euler, has_dq);
+    hipLaunchKernelGGL((k_col_upd<N, E, SB>), grid, dim3(S::COLT), 0, st, d, euler, has_dq);
     return hipGetLastError();
   }
   template <int E, int WG>
@@ -428,16 +447,16 @@ struct LS {
     hipLaunchKernelGGL((k_row_qw<N, E, RP, 2>), dim3(d.g.n2 * d.g.nz / RP), dim3(RP * (N / E)), 0, st, d, scale);
     return hipGetLastError();
   }
-  template <int E>
+  template <int E, int SB>
   static hipError_t col_qw_e(const Dev& d, hipStream_t st) {
     using S = SH<N, E>;
     dim3 grid((d.g.KX + S::TX - 1) / S::TX, d.g.nz);
-    hipLaunchKernelGGL((k_col_qw<N, E>), grid, dim3(S::COLT), 0, st, d);
+    hipLaunchKernelGGL((k_col_qw<N, E, SB>), grid, dim3(S::COLT), 0, st, d);
     return hipGetLastError();
   }
   static hipError_t ps_inv(const Dev& d, hipStream_t st, const double2* F, double2* o0, double2* o1, double2* o2,
                            int W, int box, int mask, int nz) {
-    return d.tune.e16[0] ? ps_inv_e<16>(d, st, F, o0, o1, o2, W, box, mask, nz)
+    return (d.tune.e16[0] & 1) ? ps_inv_e<16>(d, st, F, o0, o1, o2, W, box, mask, nz)
                          : ps_inv_e<8>(d, st, F, o0, o1, o2, W, box, mask, nz);
   }
   // row variants: 0 = E8 in 64-thread blocks, 1 = E16/64, 2 = E8/128,
@@ -450,8 +469,15 @@ struct LS {
       default: return row_nl_e<8, 64>(d, st, Win);
     }
   }
+  // column variants: 0 = E8, 1 = E16 (two exchange buffers), 2 = E8, 3 = E16
+  // (one exchange buffer: half the LDS, twice the occupancy, more barriers)
   static hipError_t col_upd(const Dev& d, hipStream_t st, int euler, int has_dq) {
-    return d.tune.e16[2] ? col_upd_e<16>(d, st, euler, has_dq) : col_upd_e<8>(d, st, euler, has_dq);
+    switch (d.tune.e16[2]) {
+      case 1: return col_upd_e<16, 2>(d, st, euler, has_dq);
+      case 2: return col_upd_e<8, 1>(d, st, euler, has_dq);
+      case 3: return col_upd_e<16, 1>(d, st, euler, has_dq);
+      default: return col_upd_e<8, 2>(d, st, euler, has_dq);
+    }
   }
   static hipError_t row_qw(const Dev& d, hipStream_t st) {
     switch (d.tune.e16[3]) {
@@ -462,7 +488,12 @@ struct LS {
     }
   }
   static hipError_t col_qw(const Dev& d, hipStream_t st) {
-    return d.tune.e16[4] ? col_qw_e<16>(d, st) : col_qw_e<8>(d, st);
+    switch (d.tune.e16[4]) {
+      case 1: return col_qw_e<16, 2>(d, st);
+      case 2: return col_qw_e<8, 1>(d, st);
+      case 3: return col_qw_e<16, 1>(d, st);
+      default: return col_qw_e<8, 2>(d, st);
+    }
   }
 };
 

@@ -351,22 +351,26 @@ class Adjoint:
     def set_fused(self, fused: bool):
         self._chk(self.lib.qg_adj_set_fused(self._h, 1 if fused else 0), "qg_adj_set_fused")
 
-    def objective(self, psi, BR, BI, nsteps: int) -> float:
+    def _out(self, arr):
+        return float(arr[0]) if self.m.members == 1 else arr
+
+    def objective(self, psi, BR, BI, nsteps: int):
+        """J (a float, or one per member for a batch)."""
         psi, BR, BI = self._phys(psi, BR, BI)
-        J = C.c_double()
-        self._chk(self.lib.qg_adj_objective(self._h, _dp(psi), _dp(BR), _dp(BI), self.m.n1, int(nsteps), C.byref(J)),
+        J = np.zeros(self.m.members)
+        self._chk(self.lib.qg_adj_objective(self._h, _dp(psi), _dp(BR), _dp(BI), self.m.n1, int(nsteps), _dp(J)),
                   "qg_adj_objective")
-        return J.value
+        return self._out(J)
 
     def tlm(self, psi, BR, BI, nsteps: int, dpsi, want_state: bool = False):
         """Returns dJ along dpsi and (optionally) the final-state perturbation
         as a (9, n3, n1, iktx) complex array (psi, q, qo, BR, BI, BRo, BIo, AR, AI)."""
         psi, BR, BI, dpsi = self._phys(psi, BR, BI, dpsi)
-        dJ = C.c_double()
+        dJ = np.zeros(self.m.members)
         st = np.empty((9,) + self.m.shape, np.complex128) if want_state else None
         self._chk(self.lib.qg_adj_tlm(self._h, _dp(psi), _dp(BR), _dp(BI), self.m.n1, int(nsteps), _dp(dpsi),
-                                      C.byref(dJ), _dp(st)), "qg_adj_tlm")
-        return dJ.value, st
+                                      _dp(dJ), _dp(st)), "qg_adj_tlm")
+        return self._out(dJ), st
 
     def apply(self, psi, BR, BI, nsteps: int, state_bar):
         """Adjoint of the map psi -> final state, applied to state_bar (9, n3, n1, iktx)."""
@@ -380,11 +384,11 @@ class Adjoint:
 
     def gradient(self, psi, BR, BI, nsteps: int):
         psi, BR, BI = self._phys(psi, BR, BI)
-        J = C.c_double()
+        J = np.zeros(self.m.members)
         g = np.empty(self.m.rshape, np.float64)
         self._chk(self.lib.qg_adj_gradient(self._h, _dp(psi), _dp(BR), _dp(BI), self.m.n1, int(nsteps),
-                                           C.byref(J), _dp(g)), "qg_adj_gradient")
-        return J.value, g
+                                           _dp(J), _dp(g)), "qg_adj_gradient")
+        return self._out(J), g
 
     def close(self):
         if self._h:

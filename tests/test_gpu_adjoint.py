@@ -145,16 +145,42 @@ def test_torch_autograd_harness():
 
 
 def test_rejects_unsupported_configurations():
-    P = make_params(16, 4, flags=qgybj.params.FIXED_FLOW)
-    m = qgybj.Member(P)
+    for flags in (qgybj.params.FIXED_FLOW, qgybj.params.PASSIVE_SCALAR, qgybj.params.LINEAR):
+        P = make_params(16, 4, flags=flags)
+        m = qgybj.Member(P)
+        with pytest.raises(qgybj.QGError):
+            qgybj.Adjoint(m)
+        m.close()
+    m = qgybj.Member(make_params(16, 4, coeffz=1.0))
     with pytest.raises(qgybj.QGError):
         qgybj.Adjoint(m)
     m.close()
-    P = make_params(16, 4)
-    b = qgybj.Member(P, members=2)
-    with pytest.raises(qgybj.QGError):
-        qgybj.Adjoint(b)
+
+
+def test_batch_gradients_equal_single_members():
+    """A batch context gives each member's objective and gradient, exactly as
+    single-member runs do."""
+    P, _, _, _ = _setup(seed=1)
+    ics = [physical_ics(P, k0=3, seed=s) for s in (11, 12, 13)]
+    stack = [np.stack([x[i] for x in ics]) for i in range(3)]
+    b = qgybj.Member(P, members=3)
+    ab = qgybj.Adjoint(b, checkpoint_every=4)
+    Jb, gb = ab.gradient(*stack, 9)
+    assert Jb.shape == (3,) and gb.shape == (3,) + psi_shape(P)
+    for j, (psi, BR, BI) in enumerate(ics):
+        m = qgybj.Member(P)
+        a = qgybj.Adjoint(m, checkpoint_every=4)
+        J, g = a.gradient(psi, BR, BI, 9)
+        assert J == Jb[j]
+        assert np.array_equal(g, gb[j])
+        a.close()
+        m.close()
+    ab.close()
     b.close()
+
+
+def psi_shape(P):
+    return (P.n3, P.n1, P.n1)
 
 
 @pytest.mark.parametrize("n1,n3,flags", [(32, 16, 0), (64, 8, qgybj.params.NO_FEEDBACK), (16, 40, 0)])

@@ -201,7 +201,8 @@ def test_vertical_partitions(n3):
 
 
 @pytest.mark.parametrize("n,n3", [(32, 4), (64, 8), (256, 4), (512, 2)])
-@pytest.mark.parametrize("variant", [(0, 0, 0, 0, 0), (1, 1, 1, 1, 1), (0, 2, 0, 2, 0), (1, 3, 1, 3, 1)])
+@pytest.mark.parametrize("variant", [(0, 0, 0, 0, 0), (1, 1, 1, 1, 1), (0, 2, 0, 2, 0), (1, 3, 1, 3, 1),
+                                     (0, 0, 2, 0, 2), (1, 1, 3, 1, 3)])
 def test_kernel_variants(n, n3, variant):
     """Every transform-kernel variant the tuner can select gives the oracle's
     result."""
