@@ -164,6 +164,23 @@ def adjoint_bench(steps: int = 200, every: int = 20, repeats: int = 2):
         J2, g = adj.gradient(psi, BR, BI, steps)
         t_g.append(time.perf_counter() - t0)
     ok = bool(np.isfinite(J) and J == J2 and np.all(np.isfinite(g)))
+    # central finite differences on 8 random directions
+    rng = np.random.default_rng(17)
+    fd_err = []
+    for _ in range(8):
+        d = rng.standard_normal(psi.shape)
+        d /= np.linalg.norm(d)
+        eps = 1e-5 * np.linalg.norm(psi)
+        fd = (adj.objective(psi + eps * d, BR, BI, steps) - adj.objective(psi - eps * d, BR, BI, steps)) / (2 * eps)
+        an = float(np.sum(g * d))
+        fd_err.append(abs(fd - an) / abs(an))
+    # tangent-linear / adjoint dot-product test on the full state map
+    L = qgybj.mask(128).astype(bool)
+    sbar = (rng.standard_normal((9,) + m.shape) + 1j * rng.standard_normal((9,) + m.shape)) * L
+    dpsi = rng.standard_normal(psi.shape)
+    _, dst = adj.tlm(psi, BR, BI, steps, dpsi, want_state=True)
+    lhs = float(np.sum(np.real(np.conj(dst) * sbar)[..., L]))
+    rhs = float(np.sum(dpsi * adj.apply(psi, BR, BI, steps, sbar)))
     adj.close()
     m.close()
     if not ok:
@@ -172,7 +189,81 @@ def adjoint_bench(steps: int = 200, every: int = 20, repeats: int = 2):
     return {"workload": f"128^2x64 FP64, exponential N^2, {steps} steps, checkpoints every {every}, "
                         "objective: deep wave energy, control: initial psi",
             "forward_s": round(tf, 4), "gradient_s": round(tg, 4), "adjoint_over_forward": round(tg / tf, 3),
-            "J": J}
+            "J": J, "fd_max_rel_err_8_directions": max(fd_err), "dot_product_rel_err": abs(lhs - rhs) / abs(lhs)}
+
+
+def large_bench(steps: int = 500, every: int = 50, batch_steps: int = 200, batch_every: int = 20):
+    """Forward + adjoint at 512^2 x 256 (exponential N^2, eddies peaked at
+    |k| = 10, checkpoints every 50 steps) with a central finite-difference
+    check along one random direction; then a batch of 4 gradients (different
+    seeds) at 256^2 x 128 in one context against 4 single-member gradients
+    (200 steps, checkpoints every 20: the intensifying |k| = 5 eddies outgrow
+    the leapfrog's stability limit at dt = 1e-3 after a few hundred steps for
+    some seeds)."""
+    import numpy as np
+    import qgybj
+    from qgybj import ic
+    from qgybj.params import make_params
+
+    out = {}
+    # dt = 2.5e-4: the |k| = 10 eddies intensify (KE grows ~2.5x in the first
+    # 400 steps), and u kmax dt must stay below 1 for the leapfrog at
+    # kmax = 170; at dt = 5e-4 the run blows up near step 420
+    P = make_params(512, 256, dt=2.5e-4, stratification="exponential")
+    psi = ic.eddy_psi(512, 256, 10, 7, Bu=P.Bu)
+    BR, BI = ic.wave_b(512, 256)
+    m = qgybj.Member(P)
+    adj = qgybj.Adjoint(m, checkpoint_every=every)
+    t0 = time.perf_counter()
+    J = adj.objective(psi, BR, BI, steps)
+    tf = time.perf_counter() - t0
+    adj.gradient(psi, BR, BI, steps)  # allocates the checkpoint storage
+    t0 = time.perf_counter()
+    J2, g = adj.gradient(psi, BR, BI, steps)
+    tg = time.perf_counter() - t0
+    d = np.random.default_rng(3).standard_normal(psi.shape)
+    d /= np.linalg.norm(d)
+    an = float(np.sum(g * d))
+    fd_err = {}
+    for rel in (1e-4, 1e-5, 1e-6):
+        eps = rel * np.linalg.norm(psi)
+        fd = (adj.objective(psi + eps * d, BR, BI, steps) - adj.objective(psi - eps * d, BR, BI, steps)) / (2 * eps)
+        fd_err[f"{rel:g}"] = abs(fd - an) / abs(an)
+    adj.close()
+    m.close()
+    out["large"] = {"workload": f"512^2x256 FP64, dt 2.5e-4, exponential N^2, eddies |k|=10, {steps} steps, checkpoints every "
+                                f"{every}", "forward_s": round(tf, 3), "gradient_s": round(tg, 3),
+                    "adjoint_over_forward": round(tg / tf, 3), "J": J, "J_same": J == J2,
+                    "fd_rel_err_by_step": fd_err}
+    del psi, BR, BI, g, d
+
+    P = make_params(256, 128, stratification="exponential")
+    ics = [(ic.eddy_psi(256, 128, 5, s, Bu=P.Bu),) + ic.wave_b(256, 128) for s in (21, 22, 23, 24)]
+    b = qgybj.Member(P, members=4)
+    ab = qgybj.Adjoint(b, checkpoint_every=batch_every)
+    stack = [np.stack([x[i] for x in ics]) for i in range(3)]
+    ab.gradient(*stack, batch_steps)  # allocates the checkpoint storage
+    t0 = time.perf_counter()
+    Jb, gb = ab.gradient(*stack, batch_steps)
+    tb = time.perf_counter() - t0
+    ab.close()
+    b.close()
+    ts = 0.0
+    same = True
+    for j, (psi, BR, BI) in enumerate(ics):
+        m = qgybj.Member(P)
+        a = qgybj.Adjoint(m, checkpoint_every=batch_every)
+        a.gradient(psi, BR, BI, batch_steps)
+        t0 = time.perf_counter()
+        J, g = a.gradient(psi, BR, BI, batch_steps)
+        ts += time.perf_counter() - t0
+        same = same and J == Jb[j] and np.array_equal(g, gb[j])
+        a.close()
+        m.close()
+    out["batch4"] = {"workload": f"4 gradients at 256^2x128 FP64 (seeds 21-24), {batch_steps} steps, checkpoints "
+                                 f"every {batch_every}", "batched_s": round(tb, 3), "sequential_s": round(ts, 3),
+                     "sequential_over_batched": round(ts / tb, 3), "identical_to_single": bool(same)}
+    return out
 
 
 def cpu_baseline(P, psi, BR, BI, nsteps: int = 2):
@@ -225,11 +316,16 @@ def main(argv=None):
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--variants", type=str, default="",
                     help="comma-separated kernel variants (qg_set_variants), default: the library's")
+    ap.add_argument("--large", action="store_true",
+                    help="only the large forward+adjoint (512^2x256) and batched-gradient (4 x 256^2x128) runs")
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run three 1 GiB device copies (HBM counter calibration under rocprofv3 --pmc)")
     a = ap.parse_args(argv)
     if a.calib_copy:
         return calib_copy()
+    if a.large:
+        print(json.dumps(large_bench()))
+        return 0
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs one member per GPU on a single MI355X; "
               "multi-GPU runs (--gpus > 1) are not supported", file=sys.stderr)
