@@ -48,6 +48,22 @@ __device__ __forceinline__ double2 twid(const double2* __restrict__ tw, int idx)
   return w;
 }
 
+// Twiddles w^r, r = 1..R-1, of one butterfly from a single table load
+// w = tw[k * TWN / (Ns R)]: w^(2a) = (w^a)^2 and w^(p+a) = w^p w^a for the
+// largest power of two p <= r, so every power is at most log2(R) products
+// from the table value (one table load per butterfly instead of R-1).
+template <int R, bool INV>
+__device__ __forceinline__ void tw_powers(const double2* __restrict__ tw, int idx, double2 (&wr)[R]) {
+  wr[0] = make_double2(1.0, 0.0);
+  if constexpr (R > 1) wr[1] = twid<INV>(tw, idx);
+#pragma unroll
+  for (int r = 2; r < R; ++r) {
+    int p = 1;
+    while (2 * p <= r) p *= 2;
+    wr[r] = (r == p) ? cmul(wr[p / 2], wr[p / 2]) : cmul(wr[p], wr[r - p]);
+  }
+}
+
 template <int N, int E, bool INV>
 __device__ __forceinline__ void rf_first(double2 (&v)[E], double2* xb, int t) {
   dft<E, INV>(v);
@@ -67,8 +83,10 @@ __device__ __forceinline__ void rf_mid(double2* xb, int t, const double2* __rest
 #pragma unroll
     for (int r = 0; r < R; ++r) w[b][r] = xb[xpad<E>(j + r * NB)];
     const int k = j % Ns;
+    double2 wr[R];
+    tw_powers<R, INV>(tw, (k * (N / (Ns * R))) * TWS, wr);
 #pragma unroll
-    for (int r = 1; r < R; ++r) w[b][r] = cmul(w[b][r], twid<INV>(tw, (k * r * (N / (Ns * R))) * TWS));
+    for (int r = 1; r < R; ++r) w[b][r] = cmul(w[b][r], wr[r]);
     dft<R, INV>(w[b]);
   }
   __syncthreads();
@@ -93,8 +111,10 @@ __device__ __forceinline__ void rf_last(double2 (&v)[E], const double2* xb, int 
 #pragma unroll
     for (int r = 0; r < R; ++r) w[r] = xb[xpad<E>(j + r * NB)];
     const int k = j % Ns;
+        double2 wr[R];
+        tw_powers<R, INV>(tw, (k * (N / (Ns * R))) * TWS, wr);
 #pragma unroll
-    for (int r = 1; r < R; ++r) w[r] = cmul(w[r], twid<INV>(tw, (k * r * (N / (Ns * R))) * TWS));
+        for (int r = 1; r < R; ++r) w[r] = cmul(w[r], wr[r]);
     dft<R, INV>(w);
 #pragma unroll
     for (int r = 0; r < R; ++r) v[b + r * BPT] = w[r];
@@ -147,8 +167,10 @@ __device__ __forceinline__ void rf_group(double2 (&v)[S][E], double2* xb, int t,
 #pragma unroll
           for (int r = 0; r < R; ++r) w[r] = xb[s * XL + xpad<E>(j + r * NB)];
           const int k = j % Ns;
+        double2 wr[R];
+        tw_powers<R, INV>(tw, (k * (N / (Ns * R))) * TWS, wr);
 #pragma unroll
-          for (int r = 1; r < R; ++r) w[r] = cmul(w[r], twid<INV>(tw, (k * r * (N / (Ns * R))) * TWS));
+        for (int r = 1; r < R; ++r) w[r] = cmul(w[r], wr[r]);
           dft<R, INV>(w);
         }
       __syncthreads();
@@ -174,8 +196,10 @@ __device__ __forceinline__ void rf_group(double2 (&v)[S][E], double2* xb, int t,
 #pragma unroll
         for (int r = 0; r < R; ++r) w[r] = xb[s * XL + xpad<E>(j + r * NB)];
         const int k = j % Ns;
+        double2 wr[R];
+        tw_powers<R, INV>(tw, (k * (N / (Ns * R))) * TWS, wr);
 #pragma unroll
-        for (int r = 1; r < R; ++r) w[r] = cmul(w[r], twid<INV>(tw, (k * r * (N / (Ns * R))) * TWS));
+        for (int r = 1; r < R; ++r) w[r] = cmul(w[r], wr[r]);
         dft<R, INV>(w);
 #pragma unroll
         for (int r = 0; r < R; ++r) v[S0 + s][b + r * BPT] = w[r];
