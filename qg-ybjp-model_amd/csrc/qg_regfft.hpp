@@ -48,14 +48,15 @@ __device__ __forceinline__ double2 twid(const double2* __restrict__ tw, int idx)
   return w;
 }
 
-// Twiddles w^r, r = 1..R-1, of one butterfly from a single table load
-// w = tw[k * TWN / (Ns R)]: w^(2a) = (w^a)^2 and w^(p+a) = w^p w^a for the
-// largest power of two p <= r, so every power is at most log2(R) products
-// from the table value (one table load per butterfly instead of R-1).
+// Twiddles w^r, r = 1..R-1, of one butterfly from its base w (a table value,
+// conjugated for the inverse): w^(2a) = (w^a)^2 and w^(p+a) = w^p w^a for
+// the largest power of two p <= r, so every power is at most log2(R)
+// products away from the table value.
 template <int R, bool INV>
-__device__ __forceinline__ void tw_powers(const double2* __restrict__ tw, int idx, double2 (&wr)[R]) {
+__device__ __forceinline__ void tw_pow(double2 w1, double2 (&wr)[R]) {
+  if (INV) w1.y = -w1.y;
   wr[0] = make_double2(1.0, 0.0);
-  if constexpr (R > 1) wr[1] = twid<INV>(tw, idx);
+  if constexpr (R > 1) wr[1] = w1;
 #pragma unroll
   for (int r = 2; r < R; ++r) {
     int p = 1;
@@ -63,6 +64,34 @@ __device__ __forceinline__ void tw_powers(const double2* __restrict__ tw, int id
     wr[r] = (r == p) ? cmul(wr[p / 2], wr[p / 2]) : cmul(wr[p], wr[r - p]);
   }
 }
+
+// Twiddle bases of thread t for the passes after the first: one table value
+// per butterfly of the middle pass (m) and of the last pass (l).  Loaded
+// once per kernel (or per transform) ahead of the first pass, so that no
+// pass waits on a table read.
+template <int N, int E>
+struct TwB {
+  using Pl = RPlan<N, E>;
+  static constexpr int RM = Pl::P == 3 ? Pl::R2 : 1;
+  static constexpr int RL = Pl::P == 2 ? Pl::R2 : (Pl::P == 3 ? Pl::R3 : 1);
+  static constexpr int BM = Pl::P == 3 ? E / RM : 0;
+  static constexpr int BL = Pl::P >= 2 ? E / RL : 0;
+  double2 m[BM > 0 ? BM : 1];
+  double2 l[BL > 0 ? BL : 1];
+  __device__ __forceinline__ void load(const double2* __restrict__ tw, int t) {
+    constexpr int T = N / E, TWS = TWN / N;
+    if constexpr (BM > 0) {
+      constexpr int Ns = Pl::R1;
+#pragma unroll
+      for (int b = 0; b < BM; ++b) m[b] = tw[(((t + T * b) % Ns) * (N / (Ns * RM))) * TWS];
+    }
+    if constexpr (BL > 0) {
+      constexpr int Ns = N / RL;
+#pragma unroll
+      for (int b = 0; b < BL; ++b) l[b] = tw[(((t + T * b) % Ns) * (N / (Ns * RL))) * TWS];
+    }
+  }
+};
 
 template <int N, int E, bool INV>
 __device__ __forceinline__ void rf_first(double2 (&v)[E], double2* xb, int t) {
@@ -74,17 +103,16 @@ __device__ __forceinline__ void rf_first(double2 (&v)[E], double2* xb, int t) {
 }
 
 template <int N, int E, int R, int Ns, bool INV>
-__device__ __forceinline__ void rf_mid(double2* xb, int t, const double2* __restrict__ tw) {
-  constexpr int T = N / E, NB = N / R, BPT = E / R, TWS = TWN / N;
+__device__ __forceinline__ void rf_mid(double2* xb, int t, const TwB<N, E>& tb) {
+  constexpr int T = N / E, NB = N / R, BPT = E / R;
   double2 w[BPT][R];
 #pragma unroll
   for (int b = 0; b < BPT; ++b) {
     const int j = t + T * b;
 #pragma unroll
     for (int r = 0; r < R; ++r) w[b][r] = xb[xpad<E>(j + r * NB)];
-    const int k = j % Ns;
     double2 wr[R];
-    tw_powers<R, INV>(tw, (k * (N / (Ns * R))) * TWS, wr);
+    tw_pow<R, INV>(tb.m[b], wr);
 #pragma unroll
     for (int r = 1; r < R; ++r) w[b][r] = cmul(w[b][r], wr[r]);
     dft<R, INV>(w[b]);
@@ -101,8 +129,8 @@ __device__ __forceinline__ void rf_mid(double2* xb, int t, const double2* __rest
 }
 
 template <int N, int E, int R, int Ns, bool INV>
-__device__ __forceinline__ void rf_last(double2 (&v)[E], const double2* xb, int t, const double2* __restrict__ tw) {
-  constexpr int T = N / E, NB = N / R, BPT = E / R, TWS = TWN / N;
+__device__ __forceinline__ void rf_last(double2 (&v)[E], const double2* xb, int t, const TwB<N, E>& tb) {
+  constexpr int T = N / E, NB = N / R, BPT = E / R;
   static_assert(Ns * R == N, "last pass");
 #pragma unroll
   for (int b = 0; b < BPT; ++b) {
@@ -110,11 +138,10 @@ __device__ __forceinline__ void rf_last(double2 (&v)[E], const double2* xb, int 
     double2 w[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) w[r] = xb[xpad<E>(j + r * NB)];
-    const int k = j % Ns;
-        double2 wr[R];
-        tw_powers<R, INV>(tw, (k * (N / (Ns * R))) * TWS, wr);
+    double2 wr[R];
+    tw_pow<R, INV>(tb.l[b], wr);
 #pragma unroll
-        for (int r = 1; r < R; ++r) w[r] = cmul(w[r], wr[r]);
+    for (int r = 1; r < R; ++r) w[r] = cmul(w[r], wr[r]);
     dft<R, INV>(w);
 #pragma unroll
     for (int r = 0; r < R; ++r) v[b + r * BPT] = w[r];
@@ -124,18 +151,24 @@ __device__ __forceinline__ void rf_last(double2 (&v)[E], const double2* xb, int 
 // In-place transform of the strided distribution v (forward exp(-i...),
 // inverse exp(+i...) unnormalised).  xb: this sequence's exchange buffer.
 template <int N, int E, bool INV>
-__device__ __forceinline__ void reg_fft(double2 (&v)[E], double2* xb, int t, const double2* __restrict__ tw) {
+__device__ __forceinline__ void reg_fft(double2 (&v)[E], double2* xb, int t, const TwB<N, E>& tb) {
   using Pl = RPlan<N, E>;
   if constexpr (Pl::P == 1) {
     dft<E, INV>(v);
   } else if constexpr (Pl::P == 2) {
     rf_first<N, E, INV>(v, xb, t);
-    rf_last<N, E, Pl::R2, Pl::R1, INV>(v, xb, t, tw);
+    rf_last<N, E, Pl::R2, Pl::R1, INV>(v, xb, t, tb);
   } else {
     rf_first<N, E, INV>(v, xb, t);
-    rf_mid<N, E, Pl::R2, Pl::R1, INV>(xb, t, tw);
-    rf_last<N, E, Pl::R3, Pl::R1 * Pl::R2, INV>(v, xb, t, tw);
+    rf_mid<N, E, Pl::R2, Pl::R1, INV>(xb, t, tb);
+    rf_last<N, E, Pl::R3, Pl::R1 * Pl::R2, INV>(v, xb, t, tb);
   }
+}
+template <int N, int E, bool INV>
+__device__ __forceinline__ void reg_fft(double2 (&v)[E], double2* xb, int t, const double2* __restrict__ tw) {
+  TwB<N, E> tb;
+  tb.load(tw, t);
+  reg_fft<N, E, INV>(v, xb, t, tb);
 }
 
 // S independent sequences transformed in groups of SB: every pass of a
@@ -143,9 +176,9 @@ __device__ __forceinline__ void reg_fft(double2 (&v)[E], double2* xb, int t, con
 // sequence overlaps the arithmetic of the others.  xb holds SB consecutive
 // exchange buffers of xbuf_len<N, E>() elements.
 template <int N, int E, int S, int SB, int S0, bool INV>
-__device__ __forceinline__ void rf_group(double2 (&v)[S][E], double2* xb, int t, const double2* __restrict__ tw) {
+__device__ __forceinline__ void rf_group(double2 (&v)[S][E], double2* xb, int t, const TwB<N, E>& tb) {
   using Pl = RPlan<N, E>;
-  constexpr int XL = xbuf_len<N, E>(), T = N / E, TWS = TWN / N;
+  constexpr int XL = xbuf_len<N, E>(), T = N / E;
   constexpr int G = (S - S0) < SB ? (S - S0) : SB;
 #pragma unroll
   for (int s = S0; s < S0 + G; ++s) dft<E, INV>(v[s]);
@@ -166,11 +199,10 @@ __device__ __forceinline__ void rf_group(double2 (&v)[S][E], double2* xb, int t,
           double2* w = &v[S0 + s][b * R];
 #pragma unroll
           for (int r = 0; r < R; ++r) w[r] = xb[s * XL + xpad<E>(j + r * NB)];
-          const int k = j % Ns;
-        double2 wr[R];
-        tw_powers<R, INV>(tw, (k * (N / (Ns * R))) * TWS, wr);
+          double2 wr[R];
+          tw_pow<R, INV>(tb.m[b], wr);
 #pragma unroll
-        for (int r = 1; r < R; ++r) w[r] = cmul(w[r], wr[r]);
+          for (int r = 1; r < R; ++r) w[r] = cmul(w[r], wr[r]);
           dft<R, INV>(w);
         }
       __syncthreads();
@@ -186,7 +218,7 @@ __device__ __forceinline__ void rf_group(double2 (&v)[S][E], double2* xb, int t,
       __syncthreads();
     }
     constexpr int R = (Pl::P == 2) ? Pl::R2 : Pl::R3;
-    constexpr int Ns = N / R, NB = N / R, BPT = E / R;
+    constexpr int NB = N / R, BPT = E / R;
 #pragma unroll
     for (int s = 0; s < G; ++s)
 #pragma unroll
@@ -195,9 +227,8 @@ __device__ __forceinline__ void rf_group(double2 (&v)[S][E], double2* xb, int t,
         double2 w[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) w[r] = xb[s * XL + xpad<E>(j + r * NB)];
-        const int k = j % Ns;
         double2 wr[R];
-        tw_powers<R, INV>(tw, (k * (N / (Ns * R))) * TWS, wr);
+        tw_pow<R, INV>(tb.l[b], wr);
 #pragma unroll
         for (int r = 1; r < R; ++r) w[r] = cmul(w[r], wr[r]);
         dft<R, INV>(w);
@@ -205,13 +236,19 @@ __device__ __forceinline__ void rf_group(double2 (&v)[S][E], double2* xb, int t,
         for (int r = 0; r < R; ++r) v[S0 + s][b + r * BPT] = w[r];
       }
   }
-  if constexpr (S0 + G < S) rf_group<N, E, S, SB, S0 + G, INV>(v, xb, t, tw);
+  if constexpr (S0 + G < S) rf_group<N, E, S, SB, S0 + G, INV>(v, xb, t, tb);
 }
 
 template <int N, int E, int S, int SB, bool INV>
+__device__ __forceinline__ void reg_fft_multi(double2 (&v)[S][E], double2* xb, int t, const TwB<N, E>& tb) {
+  rf_group<N, E, S, SB, 0, INV>(v, xb, t, tb);
+}
+template <int N, int E, int S, int SB, bool INV>
 __device__ __forceinline__ void reg_fft_multi(double2 (&v)[S][E], double2* xb, int t,
                                               const double2* __restrict__ tw) {
-  rf_group<N, E, S, SB, 0, INV>(v, xb, t, tw);
+  TwB<N, E> tb;
+  tb.load(tw, t);
+  rf_group<N, E, S, SB, 0, INV>(v, xb, t, tb);
 }
 
 // Publish sequences S0 .. S0+G-1 of v into buffers 0 .. G-1 so that any
