@@ -33,8 +33,8 @@ __device__ __forceinline__ double hweight(int kx, int ky) { return (kx == 0 && k
 // realit fill of the kx = 0 column for sequences S0 .. S0+G-1 of v (after
 // their multipliers): entries iky > N/2 become conj of entry N - iky
 template <int N, int E, int S, int S0, int G>
-__device__ __forceinline__ void realit_col0(double2 (&v)[S][E], double2* xb, int t, int c) {
-  constexpr int T = N / E, XL = xbuf_len<N, E>();
+__device__ __forceinline__ void realit_col0(double2 (&v)[S][E], const ColBuf<N, E>& xb, int t, int c) {
+  constexpr int T = N / E;
   rf_publish_range<N, E, S, S0, G>(v, xb, t);
   if (c == 0) {
 #pragma unroll
@@ -42,7 +42,7 @@ __device__ __forceinline__ void realit_col0(double2 (&v)[S][E], double2* xb, int
 #pragma unroll
       for (int m = 0; m < E; ++m) {
         const int iky = t + T * m;
-        if (iky > N / 2) v[S0 + s][m] = cconj(xb[s * XL + xpad<E>(N - iky)]);
+        if (iky > N / 2) v[S0 + s][m] = cconj(xb(s, N - iky));
       }
   }
 }
@@ -52,12 +52,12 @@ __device__ __forceinline__ void realit_col0(double2 (&v)[S][E], double2* xb, int
 // ---------------------------------------------------------------------------
 template <int N, int E>
 __global__ __launch_bounds__((SH<N, E>::COLT)) void k_adj_col_qw(Dev d, AdjBufs b) {
-  constexpr int T = SH<N, E>::T, TX = SH<N, E>::TX, XL = xbuf_len<N, E>();
-  __shared__ double2 lds[TX * 2 * XL];
+  constexpr int T = SH<N, E>::T, TX = SH<N, E>::TX;
+  __shared__ double2 lds[col_xlen<N, E, 2, TX>()];
   const Geo& g = d.g;
   const int c = threadIdx.x % TX, t = threadIdx.x / TX;
   const int kx0 = blockIdx.x * TX, kx = kx0 + c, z = blockIdx.y;
-  double2* xb = lds + c * 2 * XL;
+  const ColBuf<N, E> xb{lds + c};
   const bool colok = kx < g.KX;
   const size_t sz = (size_t)z * g.ikty * g.sp, pz = (size_t)z * g.n2 * g.sp;
   double2 v[2][E];
@@ -96,13 +96,14 @@ __global__ __launch_bounds__((SH<N, E>::COLT)) void k_adj_col_qw(Dev d, AdjBufs 
 // ---------------------------------------------------------------------------
 template <int N, int E, int RP, int SB>
 __global__ __launch_bounds__(RP*(N / E)) void k_adj_row_qw(Dev d, AdjBufs b, double cJ, double cE) {
-  constexpr int T = N / E, XL = xbuf_len<N, E>();
+  constexpr int T = N / E, RS = row_xstride<N, E, SB>();
   constexpr int MK0 = ((N / 3 + 2) + T - 1) / T;
   constexpr int MK = MK0 < E ? MK0 : E;
-  __shared__ double2 lds[RP * SB * XL];
+  __shared__ double2 lds[RP * RS];
   const Geo& g = d.g;
-  const int t = threadIdx.x % T, p = threadIdx.x / T;
-  double2* xb = lds + p * SB * XL;
+  int t, p;
+  row_lane<T, RP>(threadIdx.x, t, p);
+  const RowBuf<N, E> xb{lds + p * RS};
   const size_t ro = ((size_t)blockIdx.x * RP + p) * g.sp;
   const int KX = g.KX;
   double2 z[4][E];
@@ -142,10 +143,10 @@ __global__ __launch_bounds__(RP*(N / E)) void k_adj_row_qw(Dev d, AdjBufs b, dou
     kx_r[m] = zero2();
     kx_i[m] = zero2();
     if (k < KX) {
-      const int km = xpad<E>((N - k) & (N - 1));
+      const int km = (N - k) & (N - 1);
       double2 Xrx, Xix, Xry, Xiy;
-      split2(pv[0][m], xb[km], Xrx, Xix);
-      split2(pv[1][m], xb[XL + km], Xry, Xiy);
+      split2(pv[0][m], xb(0, km), Xrx, Xix);
+      split2(pv[1][m], xb(1, km), Xry, Xiy);
       const double kk = (double)k;
       kx_r[m] = mul_ik(Xrx, -kk);
       kx_i[m] = mul_ik(Xix, -kk);
@@ -159,7 +160,7 @@ __global__ __launch_bounds__(RP*(N / E)) void k_adj_row_qw(Dev d, AdjBufs b, dou
     const int k = t + T * m;
     if (k < KX) {
       double2 Xr, Xi;
-      split2(pv[2][m], xb[xpad<E>((N - k) & (N - 1))], Xr, Xi);
+      split2(pv[2][m], xb(0, (N - k) & (N - 1)), Xr, Xi);
       b.A[2][ro + k] = cadd(kx_r[m], Xr);
       b.A[4][ro + k] = cadd(kx_i[m], Xi);
     }
@@ -175,12 +176,12 @@ __global__ __launch_bounds__(RP*(N / E)) void k_adj_row_qw(Dev d, AdjBufs b, dou
 // ---------------------------------------------------------------------------
 template <int N, int E>
 __global__ __launch_bounds__((SH<N, E>::COLT)) void k_adj_col_upd(Dev d, AdjBufs b, int euler, int feedback) {
-  constexpr int T = SH<N, E>::T, TX = SH<N, E>::TX, XL = xbuf_len<N, E>();
-  __shared__ double2 lds[TX * 2 * XL];
+  constexpr int T = SH<N, E>::T, TX = SH<N, E>::TX;
+  __shared__ double2 lds[col_xlen<N, E, 2, TX>()];
   const Geo& g = d.g;
   const int c = threadIdx.x % TX, t = threadIdx.x / TX;
   const int kx0 = blockIdx.x * TX, kx = kx0 + c, z = blockIdx.y, f = blockIdx.z;
-  double2* xb = lds + c * 2 * XL;
+  const ColBuf<N, E> xb{lds + c};
   const bool colok = kx < g.KX;
   const size_t sz = (size_t)z * g.ikty * g.sp, pz = (size_t)z * g.n2 * g.sp;
   const int toff = t * g.sp + kx;
@@ -279,13 +280,14 @@ __global__ __launch_bounds__((SH<N, E>::COLT)) void k_adj_col_upd(Dev d, AdjBufs
 // ---------------------------------------------------------------------------
 template <int N, int E, int RP, int SB>
 __global__ __launch_bounds__(RP*(N / E)) void k_adj_row_nl(Dev d, AdjBufs b, int Win, double s1, double s2) {
-  constexpr int T = N / E, XL = xbuf_len<N, E>();
+  constexpr int T = N / E, RS = row_xstride<N, E, SB>();
   constexpr int MK0 = ((N / 3 + 2) + T - 1) / T;
   constexpr int MK = MK0 < E ? MK0 : E;
-  __shared__ double2 lds[RP * SB * XL];
+  __shared__ double2 lds[RP * RS];
   const Geo& g = d.g;
-  const int t = threadIdx.x % T, p = threadIdx.x / T;
-  double2* xb = lds + p * SB * XL;
+  int t, p;
+  row_lane<T, RP>(threadIdx.x, t, p);
+  const RowBuf<N, E> xb{lds + p * RS};
   const size_t ro = ((size_t)blockIdx.x * RP + p) * g.sp;
   const int KX = g.KX;
   double2 z[7][E];
@@ -338,10 +340,10 @@ __global__ __launch_bounds__(RP*(N / E)) void k_adj_row_nl(Dev d, AdjBufs b, int
   for (int m = 0; m < MK; ++m) {
     const int k = t + T * m;
     if (k < KX) {
-      const int km = xpad<E>((N - k) & (N - 1));
+      const int km = (N - k) & (N - 1);
       double2 Xu, Xv, Xz, Xq;
-      split2(pv[0][m], xb[km], Xu, Xv);
-      split2(pv[1][m], xb[XL + km], Xz, Xq);
+      split2(pv[0][m], xb(0, km), Xu, Xv);
+      split2(pv[1][m], xb(1, km), Xz, Xq);
       const double kk = (double)k;
       const double2 a1 = mul_ik(Xv, -kk);
       b.A[0][ro + k] = Xu;
@@ -356,7 +358,7 @@ __global__ __launch_bounds__(RP*(N / E)) void k_adj_row_nl(Dev d, AdjBufs b, int
     const int k = t + T * m;
     if (k < KX) {
       double2 Xr, Xi;
-      split2(pv[2][m], xb[xpad<E>((N - k) & (N - 1))], Xr, Xi);
+      split2(pv[2][m], xb(0, (N - k) & (N - 1)), Xr, Xi);
       b.A[4][ro + k] = Xr;
       b.A[5][ro + k] = Xi;
     }
@@ -371,12 +373,12 @@ __global__ __launch_bounds__(RP*(N / E)) void k_adj_row_nl(Dev d, AdjBufs b, int
 // ---------------------------------------------------------------------------
 template <int N, int E>
 __global__ __launch_bounds__((SH<N, E>::COLT)) void k_adj_col_out(Dev d, AdjBufs b) {
-  constexpr int T = SH<N, E>::T, TX = SH<N, E>::TX, XL = xbuf_len<N, E>();
-  __shared__ double2 lds[TX * 2 * XL];
+  constexpr int T = SH<N, E>::T, TX = SH<N, E>::TX;
+  __shared__ double2 lds[col_xlen<N, E, 2, TX>()];
   const Geo& g = d.g;
   const int c = threadIdx.x % TX, t = threadIdx.x / TX;
   const int kx0 = blockIdx.x * TX, kx = kx0 + c, z = blockIdx.y, f = blockIdx.z;
-  double2* xb = lds + c * 2 * XL;
+  const ColBuf<N, E> xb{lds + c};
   const bool colok = kx < g.KX;
   const size_t sz = (size_t)z * g.ikty * g.sp, pz = (size_t)z * g.n2 * g.sp;
   const int toff = t * g.sp + kx;

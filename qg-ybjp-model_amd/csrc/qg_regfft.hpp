@@ -9,8 +9,8 @@
 // into registers at exactly the strided positions, so that a pointwise
 // operation (products, the time update) can run between two transforms with
 // no data movement.  A 2-pass transform costs one LDS write and one LDS read
-// per element.  LDS addresses carry one pad element per E elements so that
-// the radix-E strided writes of the first pass are free of bank conflicts.
+// per element; the buffer layout (XBuf) keeps every exchange free of LDS
+// bank conflicts.
 //
 // All threads of the workgroup must call reg_fft together (it contains
 // __syncthreads); every thread owns one sequence position set.
@@ -34,12 +34,64 @@ template <> struct RPlan<256, 8>  { static constexpr int P = 3, R1 = 8,  R2 = 8,
 template <> struct RPlan<512, 16> { static constexpr int P = 3, R1 = 16, R2 = 8, R3 = 4; };
 template <> struct RPlan<512, 8>  { static constexpr int P = 3, R1 = 8,  R2 = 8, R3 = 8; };
 
-// padded LDS index and the size of one sequence's exchange buffer; the +1
-// keeps consecutive buffers (one per column) on different banks
-template <int E>
-__device__ __forceinline__ int xpad(int i) { return i + i / E; }
+// LDS exchange buffers as seen by one thread.  gfx950 serves a b128 LDS read
+// 16 lanes per cycle over 64 four-byte banks (16 distinct 16-byte slots) and a
+// b128 write 8 lanes per cycle over 32 banks (8 slots); two lanes of one
+// group on the same slot at different addresses cost an extra cycle.
+//  - ST == 1 (row kernels): the buffers of one sequence group are contiguous,
+//    S buffers of SL elements with one pad element per E, so that the
+//    stride-E writes of the first pass fall on 8 distinct slots; rows are
+//    paired lane-wise (row_lane) and spaced row_xstride() apart, which is 8
+//    slots modulo 16, so that a 16-lane read of 8 consecutive positions of
+//    two rows is conflict-free.
+//  - ST > 1 (column kernels, ST = TX columns): the TX columns of a workgroup
+//    are interleaved element by element with no padding: the 8 lanes of a
+//    write are 8 columns (distinct slots) and the 16 lanes of a read are 8
+//    columns at two consecutive positions (8 slots apart).
+// Every offset that is known at compile time stays additive, so LDS
+// instructions keep immediate offsets.
+template <int N, int E, int ST>
+struct XBuf {
+  static constexpr int SL = ST == 1 ? N + N / E : N;
+  double2* p;
+  __device__ __forceinline__ double2& operator()(int s, int i) const {
+    if constexpr (ST == 1) return p[s * SL + i + i / E];
+    else return p[(s * SL + i) * ST];
+  }
+};
 template <int N, int E>
-constexpr int xbuf_len() { return N + N / E + 1; }
+using RowBuf = XBuf<N, E, 1>;
+template <int N, int E>
+using ColBuf = XBuf<N, E, ((64 / (N / E)) > 8 ? 64 / (N / E) : 8)>;
+
+// Elements of LDS a row workgroup with RP rows needs for SB buffers per row,
+// and the spacing of consecutive rows (8 slots modulo 16).
+template <int N, int E, int SB>
+constexpr int row_xstride() {
+  constexpr int L = SB * RowBuf<N, E>::SL;
+  return L + ((8 - L) % 16 + 16) % 16;
+}
+// Elements of LDS a column workgroup needs for SB buffers per column.
+template <int N, int E, int SB, int TX>
+constexpr int col_xlen() { return TX * SB * N; }
+
+// Lane -> (position t, row p) of a row workgroup of RP rows with T threads
+// per row.  Lanes go in chunks of 8 consecutive positions; chunks of rows 2a
+// and 2a+1 alternate, so each 16-lane group reads 8 positions of two rows and
+// each 8-lane group stays inside one row (one 128-byte line of global
+// memory).  Odd RP or T < 8 keep the plain mapping.
+template <int T, int RP>
+__device__ __forceinline__ void row_lane(int tid, int& t, int& p) {
+  if constexpr (RP % 2 != 0 || T < 8) {
+    t = tid % T;
+    p = tid / T;
+  } else {
+    constexpr int C = T / 8;
+    const int ch = tid >> 3;
+    t = (tid & 7) + 8 * ((ch >> 1) % C);
+    p = 2 * ((ch >> 1) / C) + (ch & 1);
+  }
+}
 
 template <bool INV>
 __device__ __forceinline__ double2 twid(const double2* __restrict__ tw, int idx) {
@@ -93,24 +145,24 @@ struct TwB {
   }
 };
 
-template <int N, int E, bool INV>
-__device__ __forceinline__ void rf_first(double2 (&v)[E], double2* xb, int t) {
+template <int N, int E, bool INV, int ST>
+__device__ __forceinline__ void rf_first(double2 (&v)[E], const XBuf<N, E, ST>& xb, int t) {
   dft<E, INV>(v);
   __syncthreads();  // the buffer may still be read by a previous transform
 #pragma unroll
-  for (int r = 0; r < E; ++r) xb[xpad<E>(t * E + r)] = v[r];
+  for (int r = 0; r < E; ++r) xb(0, t * E + r) = v[r];
   __syncthreads();
 }
 
-template <int N, int E, int R, int Ns, bool INV>
-__device__ __forceinline__ void rf_mid(double2* xb, int t, const TwB<N, E>& tb) {
+template <int N, int E, int R, int Ns, bool INV, int ST>
+__device__ __forceinline__ void rf_mid(const XBuf<N, E, ST>& xb, int t, const TwB<N, E>& tb) {
   constexpr int T = N / E, NB = N / R, BPT = E / R;
   double2 w[BPT][R];
 #pragma unroll
   for (int b = 0; b < BPT; ++b) {
     const int j = t + T * b;
 #pragma unroll
-    for (int r = 0; r < R; ++r) w[b][r] = xb[xpad<E>(j + r * NB)];
+    for (int r = 0; r < R; ++r) w[b][r] = xb(0, j + r * NB);
     double2 wr[R];
     tw_pow<R, INV>(tb.m[b], wr);
 #pragma unroll
@@ -123,13 +175,13 @@ __device__ __forceinline__ void rf_mid(double2* xb, int t, const TwB<N, E>& tb) 
     const int j = t + T * b;
     const int base = (j / Ns) * Ns * R + (j % Ns);
 #pragma unroll
-    for (int r = 0; r < R; ++r) xb[xpad<E>(base + r * Ns)] = w[b][r];
+    for (int r = 0; r < R; ++r) xb(0, base + r * Ns) = w[b][r];
   }
   __syncthreads();
 }
 
-template <int N, int E, int R, int Ns, bool INV>
-__device__ __forceinline__ void rf_last(double2 (&v)[E], const double2* xb, int t, const TwB<N, E>& tb) {
+template <int N, int E, int R, int Ns, bool INV, int ST>
+__device__ __forceinline__ void rf_last(double2 (&v)[E], const XBuf<N, E, ST>& xb, int t, const TwB<N, E>& tb) {
   constexpr int T = N / E, NB = N / R, BPT = E / R;
   static_assert(Ns * R == N, "last pass");
 #pragma unroll
@@ -137,7 +189,7 @@ __device__ __forceinline__ void rf_last(double2 (&v)[E], const double2* xb, int 
     const int j = t + T * b;
     double2 w[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) w[r] = xb[xpad<E>(j + r * NB)];
+    for (int r = 0; r < R; ++r) w[r] = xb(0, j + r * NB);
     double2 wr[R];
     tw_pow<R, INV>(tb.l[b], wr);
 #pragma unroll
@@ -150,8 +202,8 @@ __device__ __forceinline__ void rf_last(double2 (&v)[E], const double2* xb, int 
 
 // In-place transform of the strided distribution v (forward exp(-i...),
 // inverse exp(+i...) unnormalised).  xb: this sequence's exchange buffer.
-template <int N, int E, bool INV>
-__device__ __forceinline__ void reg_fft(double2 (&v)[E], double2* xb, int t, const TwB<N, E>& tb) {
+template <int N, int E, bool INV, int ST>
+__device__ __forceinline__ void reg_fft(double2 (&v)[E], const XBuf<N, E, ST>& xb, int t, const TwB<N, E>& tb) {
   using Pl = RPlan<N, E>;
   if constexpr (Pl::P == 1) {
     dft<E, INV>(v);
@@ -164,8 +216,9 @@ __device__ __forceinline__ void reg_fft(double2 (&v)[E], double2* xb, int t, con
     rf_last<N, E, Pl::R3, Pl::R1 * Pl::R2, INV>(v, xb, t, tb);
   }
 }
-template <int N, int E, bool INV>
-__device__ __forceinline__ void reg_fft(double2 (&v)[E], double2* xb, int t, const double2* __restrict__ tw) {
+template <int N, int E, bool INV, int ST>
+__device__ __forceinline__ void reg_fft(double2 (&v)[E], const XBuf<N, E, ST>& xb, int t,
+                                        const double2* __restrict__ tw) {
   TwB<N, E> tb;
   tb.load(tw, t);
   reg_fft<N, E, INV>(v, xb, t, tb);
@@ -173,12 +226,13 @@ __device__ __forceinline__ void reg_fft(double2 (&v)[E], double2* xb, int t, con
 
 // S independent sequences transformed in groups of SB: every pass of a
 // group runs between the same two barriers, so the LDS latency of one
-// sequence overlaps the arithmetic of the others.  xb holds SB consecutive
-// exchange buffers of xbuf_len<N, E>() elements.
-template <int N, int E, int S, int SB, int S0, bool INV>
-__device__ __forceinline__ void rf_group(double2 (&v)[S][E], double2* xb, int t, const TwB<N, E>& tb) {
+// sequence overlaps the arithmetic of the others.  xb holds SB exchange
+// buffers.
+template <int N, int E, int S, int SB, int S0, bool INV, int ST>
+__device__ __forceinline__ void rf_group(double2 (&v)[S][E], const XBuf<N, E, ST>& xb, int t,
+                                         const TwB<N, E>& tb) {
   using Pl = RPlan<N, E>;
-  constexpr int XL = xbuf_len<N, E>(), T = N / E;
+  constexpr int T = N / E;
   constexpr int G = (S - S0) < SB ? (S - S0) : SB;
 #pragma unroll
   for (int s = S0; s < S0 + G; ++s) dft<E, INV>(v[s]);
@@ -187,7 +241,7 @@ __device__ __forceinline__ void rf_group(double2 (&v)[S][E], double2* xb, int t,
 #pragma unroll
     for (int s = 0; s < G; ++s)
 #pragma unroll
-      for (int r = 0; r < E; ++r) xb[s * XL + xpad<E>(t * E + r)] = v[S0 + s][r];
+      for (int r = 0; r < E; ++r) xb(s, t * E + r) = v[S0 + s][r];
     __syncthreads();
     if constexpr (Pl::P == 3) {
       constexpr int R = Pl::R2, Ns = Pl::R1, NB = N / R, BPT = E / R;
@@ -198,7 +252,7 @@ __device__ __forceinline__ void rf_group(double2 (&v)[S][E], double2* xb, int t,
           const int j = t + T * b;
           double2* w = &v[S0 + s][b * R];
 #pragma unroll
-          for (int r = 0; r < R; ++r) w[r] = xb[s * XL + xpad<E>(j + r * NB)];
+          for (int r = 0; r < R; ++r) w[r] = xb(s, j + r * NB);
           double2 wr[R];
           tw_pow<R, INV>(tb.m[b], wr);
 #pragma unroll
@@ -213,7 +267,7 @@ __device__ __forceinline__ void rf_group(double2 (&v)[S][E], double2* xb, int t,
           const int j = t + T * b;
           const int base = (j / Ns) * Ns * R + (j % Ns);
 #pragma unroll
-          for (int r = 0; r < R; ++r) xb[s * XL + xpad<E>(base + r * Ns)] = v[S0 + s][b * R + r];
+          for (int r = 0; r < R; ++r) xb(s, base + r * Ns) = v[S0 + s][b * R + r];
         }
       __syncthreads();
     }
@@ -226,7 +280,7 @@ __device__ __forceinline__ void rf_group(double2 (&v)[S][E], double2* xb, int t,
         const int j = t + T * b;
         double2 w[R];
 #pragma unroll
-        for (int r = 0; r < R; ++r) w[r] = xb[s * XL + xpad<E>(j + r * NB)];
+        for (int r = 0; r < R; ++r) w[r] = xb(s, j + r * NB);
         double2 wr[R];
         tw_pow<R, INV>(tb.l[b], wr);
 #pragma unroll
@@ -239,12 +293,13 @@ __device__ __forceinline__ void rf_group(double2 (&v)[S][E], double2* xb, int t,
   if constexpr (S0 + G < S) rf_group<N, E, S, SB, S0 + G, INV>(v, xb, t, tb);
 }
 
-template <int N, int E, int S, int SB, bool INV>
-__device__ __forceinline__ void reg_fft_multi(double2 (&v)[S][E], double2* xb, int t, const TwB<N, E>& tb) {
+template <int N, int E, int S, int SB, bool INV, int ST>
+__device__ __forceinline__ void reg_fft_multi(double2 (&v)[S][E], const XBuf<N, E, ST>& xb, int t,
+                                              const TwB<N, E>& tb) {
   rf_group<N, E, S, SB, 0, INV>(v, xb, t, tb);
 }
-template <int N, int E, int S, int SB, bool INV>
-__device__ __forceinline__ void reg_fft_multi(double2 (&v)[S][E], double2* xb, int t,
+template <int N, int E, int S, int SB, bool INV, int ST>
+__device__ __forceinline__ void reg_fft_multi(double2 (&v)[S][E], const XBuf<N, E, ST>& xb, int t,
                                               const double2* __restrict__ tw) {
   TwB<N, E> tb;
   tb.load(tw, t);
@@ -255,28 +310,28 @@ __device__ __forceinline__ void reg_fft_multi(double2 (&v)[S][E], double2* xb, i
 // position can be read (the k <-> N-k pairs of two-for-one real transforms,
 // the realit fill); the caller synchronises before the buffers are written
 // again.
-template <int N, int E, int S, int S0, int G>
-__device__ __forceinline__ void rf_publish_range(const double2 (&v)[S][E], double2* xb, int t) {
-  constexpr int T = N / E, XL = xbuf_len<N, E>();
+template <int N, int E, int S, int S0, int G, int ST>
+__device__ __forceinline__ void rf_publish_range(const double2 (&v)[S][E], const XBuf<N, E, ST>& xb, int t) {
+  constexpr int T = N / E;
   __syncthreads();
 #pragma unroll
   for (int s = 0; s < G; ++s)
 #pragma unroll
-    for (int m = 0; m < E; ++m) xb[s * XL + xpad<E>(t + T * m)] = v[S0 + s][m];
+    for (int m = 0; m < E; ++m) xb(s, t + T * m) = v[S0 + s][m];
   __syncthreads();
 }
 
 // Exchange the strided distribution through the buffer so that every thread
 // can read arbitrary positions of the sequence (used for the k <-> N-k pairs
 // of two-for-one real transforms and for the realit fill).  After the call,
-// xb[xpad<E>(i)] holds element i; the caller synchronises before the buffer
-// is written again.
-template <int N, int E>
-__device__ __forceinline__ void rf_publish(const double2 (&v)[E], double2* xb, int t) {
+// xb(0, i) holds element i; the caller synchronises before the buffer is
+// written again.
+template <int N, int E, int ST>
+__device__ __forceinline__ void rf_publish(const double2 (&v)[E], const XBuf<N, E, ST>& xb, int t) {
   constexpr int T = N / E;
   __syncthreads();
 #pragma unroll
-  for (int m = 0; m < E; ++m) xb[xpad<E>(t + T * m)] = v[m];
+  for (int m = 0; m < E; ++m) xb(0, t + T * m) = v[m];
   __syncthreads();
 }
 

@@ -44,11 +44,11 @@ __global__ __launch_bounds__((SH<N, E>::COLT)) void k_ps_inv(Geo g, const double
                                                                    const double2* __restrict__ F, double2* o0,
                                                                    double2* o1, double2* o2, int W, int box,
                                                                    int mask) {
-  constexpr int T = SH<N, E>::T, TX = SH<N, E>::TX, XL = xbuf_len<N, E>();
-  __shared__ double2 lds[TX * XL];
+  constexpr int T = SH<N, E>::T, TX = SH<N, E>::TX;
+  __shared__ double2 lds[col_xlen<N, E, 1, TX>()];
   const int c = threadIdx.x % TX, t = threadIdx.x / TX;
   const int kx0 = blockIdx.x * TX, kx = kx0 + c, z = blockIdx.y;
-  double2* xb = lds + c * XL;
+  const ColBuf<N, E> xb{lds + c};
   const double2* Fz = F + (size_t)z * g.ikty * g.sp;
   double2 v0[E];
 #pragma unroll
@@ -74,7 +74,7 @@ __global__ __launch_bounds__((SH<N, E>::COLT)) void k_ps_inv(Geo g, const double
 #pragma unroll
         for (int m = 0; m < E; ++m) {
           const int iky = t + T * m;
-          if (iky > N / 2) v[m] = cconj(xb[xpad<E>(N - iky)]);
+          if (iky > N / 2) v[m] = cconj(xb(0, N - iky));
         }
       }
     }
@@ -98,25 +98,37 @@ __global__ __launch_bounds__((SH<N, E>::COLT)) void k_ps_inv(Geo g, const double
 // ---------------------------------------------------------------------------
 template <int N, int E, int RP, int SB>
 __global__ __launch_bounds__(RP*(N / E)) void k_row_nl(Dev d, int Win, double scale) {
-  constexpr int T = N / E, XL = xbuf_len<N, E>();
+  constexpr int T = N / E, RS = row_xstride<N, E, SB>();
   constexpr int MK0 = ((N / 3 + 2) + T - 1) / T;
   constexpr int MK = MK0 < E ? MK0 : E;  // positions t + T*m that can be < KX
-  __shared__ double2 lds[RP * SB * XL];
+  __shared__ double2 lds[RP * RS];
   const Geo& g = d.g;
-  const int t = threadIdx.x % T, p = threadIdx.x / T;
-  double2* xb = lds + p * SB * XL;
+  int t, p;
+  row_lane<T, RP>(threadIdx.x, t, p);
+  const RowBuf<N, E> xb{lds + p * RS};
   const size_t ro = ((size_t)blockIdx.x * RP + p) * g.sp;
   double2* const* I = d.I;
+  // All loads are issued before any is used: positions with k >= Win read
+  // entry 0 of the row (a valid address, same line as the others) and are
+  // zeroed afterwards, so no branch splits the loads (a zero-initialised
+  // branch makes the compiler wait on each load to copy it).
+  double2 in[6][E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int pos = t + T * m;
+    const int k = pos <= N / 2 ? pos : N - pos;
+    const size_t a = ro + (k < Win ? k : 0);
+#pragma unroll
+    for (int f = 0; f < 6; ++f) in[f][m] = I[f][a];
+  }
   double2 z[3][E];
 #pragma unroll
   for (int m = 0; m < E; ++m) {
     const int pos = t + T * m;
     const int k = pos <= N / 2 ? pos : N - pos;
-    double2 ps = zero2(), py = zero2(), pyy = zero2(), pq = zero2(), pbr = zero2(), pbi = zero2();
-    if (k < Win) {
-      ps = I[1][ro + k]; py = I[3][ro + k]; pyy = I[5][ro + k];
-      pq = I[0][ro + k]; pbr = I[2][ro + k]; pbi = I[4][ro + k];
-    }
+    const bool ok = k < Win;
+    const double2 pq = ok ? in[0][m] : zero2(), ps = ok ? in[1][m] : zero2(), pbr = ok ? in[2][m] : zero2();
+    const double2 py = ok ? in[3][m] : zero2(), pbi = ok ? in[4][m] : zero2(), pyy = ok ? in[5][m] : zero2();
     const double kk = (double)k, k2 = kk * kk;
     const double2 U = make_double2(-py.x, -py.y);
     const double2 V = mul_ik(ps, kk);
@@ -149,10 +161,10 @@ __global__ __launch_bounds__(RP*(N / E)) void k_row_nl(Dev d, int Win, double sc
     const int k = t + T * m;
     keepZR[m] = zero2();
     if (k < KX) {
-      const int km = xpad<E>((N - k) & (N - 1));
+      const int km = (N - k) & (N - 1);
       double2 FzR, FzI, FuR, FvR;
-      split2(pv[0][m], xb[km], FzR, FzI);
-      split2(pv[1][m], xb[XL + km], FuR, FvR);
+      split2(pv[0][m], xb(0, km), FzR, FzI);
+      split2(pv[1][m], xb(1, km), FuR, FvR);
       double2 w2 = mul_ik(FuR, (double)k), w3 = FvR;
       if (lin) { w2 = zero2(); w3 = zero2(); }
       if (!pas) w2 = make_double2(w2.x - 0.5 * FzI.x, w2.y - 0.5 * FzI.y);
@@ -168,10 +180,10 @@ __global__ __launch_bounds__(RP*(N / E)) void k_row_nl(Dev d, int Win, double sc
   for (int m = 0; m < MK; ++m) {
     const int k = t + T * m;
     if (k < KX) {
-      const int km = xpad<E>((N - k) & (N - 1));
+      const int km = (N - k) & (N - 1);
       double2 FuI, FvI, Fuq, Fvq;
-      split2(pv[2][m], xb[km], FuI, FvI);
-      split2(pv[3][m], xb[XL + km], Fuq, Fvq);
+      split2(pv[2][m], xb(0, km), FuI, FvI);
+      split2(pv[3][m], xb(1, km), Fuq, Fvq);
       const double kk = (double)k;
       double2 w0 = mul_ik(Fuq, kk), w1 = Fvq, w4 = mul_ik(FuI, kk), w5 = FvI;
       if (lin) { w0 = zero2(); w1 = zero2(); w4 = zero2(); w5 = zero2(); }
@@ -191,12 +203,12 @@ __global__ __launch_bounds__(RP*(N / E)) void k_row_nl(Dev d, int Win, double sc
 // ---------------------------------------------------------------------------
 template <int N, int E, int SB>
 __global__ __launch_bounds__((SH<N, E>::COLT)) void k_col_upd(Dev d, int euler, int has_dq) {
-  constexpr int T = SH<N, E>::T, TX = SH<N, E>::TX, XL = xbuf_len<N, E>();
-  __shared__ double2 lds[TX * SB * XL];
+  constexpr int T = SH<N, E>::T, TX = SH<N, E>::TX;
+  __shared__ double2 lds[col_xlen<N, E, SB, TX>()];
   const Geo& g = d.g;
   const int c = threadIdx.x % TX, t = threadIdx.x / TX;
   const int kx0 = blockIdx.x * TX, kx = kx0 + c, z = blockIdx.y, f = blockIdx.z;
-  double2* xb = lds + c * SB * XL;
+  const ColBuf<N, E> xb{lds + c};
   double2* W1 = d.I[2 * f];
   double2* W2 = d.I[2 * f + 1];
   const size_t pz = (size_t)z * g.n2 * g.sp;
@@ -284,8 +296,8 @@ __global__ __launch_bounds__((SH<N, E>::COLT)) void k_col_upd(Dev d, int euler, 
         for (int m = 0; m < E; ++m) {
           const int iky = t + T * m;
           if (iky > N / 2) {
-            v[0][m] = cconj(xb[xpad<E>(N - iky)]);
-            v[1][m] = cconj(xb[XL + xpad<E>(N - iky)]);
+            v[0][m] = cconj(xb(0, N - iky));
+            v[1][m] = cconj(xb(1, N - iky));
           }
         }
       }
@@ -295,7 +307,7 @@ __global__ __launch_bounds__((SH<N, E>::COLT)) void k_col_upd(Dev d, int euler, 
 #pragma unroll
         for (int m = 0; m < E; ++m) {
           const int iky = t + T * m;
-          if (iky > N / 2) v[0][m] = cconj(xb[xpad<E>(N - iky)]);
+          if (iky > N / 2) v[0][m] = cconj(xb(0, N - iky));
         }
       }
       rf_publish_range<N, E, 2, 1, 1>(v, xb, t);
@@ -303,7 +315,7 @@ __global__ __launch_bounds__((SH<N, E>::COLT)) void k_col_upd(Dev d, int euler, 
 #pragma unroll
         for (int m = 0; m < E; ++m) {
           const int iky = t + T * m;
-          if (iky > N / 2) v[1][m] = cconj(xb[xpad<E>(N - iky)]);
+          if (iky > N / 2) v[1][m] = cconj(xb(0, N - iky));
         }
       }
     }
@@ -326,13 +338,14 @@ __global__ __launch_bounds__((SH<N, E>::COLT)) void k_col_upd(Dev d, int euler, 
 // ---------------------------------------------------------------------------
 template <int N, int E, int RP, int SB>
 __global__ __launch_bounds__(RP*(N / E)) void k_row_qw(Dev d, double scale) {
-  constexpr int T = N / E, XL = xbuf_len<N, E>();
+  constexpr int T = N / E, RS = row_xstride<N, E, SB>();
   constexpr int MK0 = ((N / 3 + 2) + T - 1) / T;
   constexpr int MK = MK0 < E ? MK0 : E;
-  __shared__ double2 lds[RP * SB * XL];
+  __shared__ double2 lds[RP * RS];
   const Geo& g = d.g;
-  const int t = threadIdx.x % T, p = threadIdx.x / T;
-  double2* xb = lds + p * SB * XL;
+  int t, p;
+  row_lane<T, RP>(threadIdx.x, t, p);
+  const RowBuf<N, E> xb{lds + p * RS};
   const size_t ro = ((size_t)blockIdx.x * RP + p) * g.sp;
   double2* const* I = d.I;
   const int KX = g.KX;
@@ -362,7 +375,7 @@ __global__ __launch_bounds__(RP*(N / E)) void k_row_qw(Dev d, double scale) {
     const int k = t + T * m;
     if (k < KX) {
       double2 FJ, FM;
-      split2(pv[0][m], xb[xpad<E>((N - k) & (N - 1))], FJ, FM);
+      split2(pv[0][m], xb(0, (N - k) & (N - 1)), FJ, FM);
       const double c4 = 0.25 * (double)k * (double)k;
       I[3][ro + k] = cscale(make_double2(FJ.x - c4 * FM.x, FJ.y - c4 * FM.y), scale);
       I[5][ro + k] = cscale(FM, scale);
@@ -377,12 +390,12 @@ __global__ __launch_bounds__(RP*(N / E)) void k_row_qw(Dev d, double scale) {
 // ---------------------------------------------------------------------------
 template <int N, int E, int SB>
 __global__ __launch_bounds__((SH<N, E>::COLT)) void k_col_qw(Dev d) {
-  constexpr int T = SH<N, E>::T, TX = SH<N, E>::TX, XL = xbuf_len<N, E>();
-  __shared__ double2 lds[TX * SB * XL];
+  constexpr int T = SH<N, E>::T, TX = SH<N, E>::TX;
+  __shared__ double2 lds[col_xlen<N, E, SB, TX>()];
   const Geo& g = d.g;
   const int c = threadIdx.x % TX, t = threadIdx.x / TX;
   const int kx0 = blockIdx.x * TX, kx = kx0 + c, z = blockIdx.y;
-  double2* xb = lds + c * SB * XL;
+  const ColBuf<N, E> xb{lds + c};
   const size_t pz = (size_t)z * g.n2 * g.sp;
   const bool colok = kx < g.KX;
   const int toff = t * g.sp + kx;
