@@ -42,7 +42,7 @@ __device__ __forceinline__ void realit_col0(double2 (&v)[S][E], const ColBuf<N, 
 #pragma unroll
       for (int m = 0; m < E; ++m) {
         const int iky = t + T * m;
-        if (iky > N / 2) v[S0 + s][m] = cconj(xb(s, N - iky));
+        if (iky > N / 2) v[S0 + s][m] = cconj(xb.pub(s, N - iky));
       }
   }
 }
@@ -145,8 +145,8 @@ __global__ __launch_bounds__(RP*(N / E)) void k_adj_row_qw(Dev d, AdjBufs b, dou
     if (k < KX) {
       const int km = (N - k) & (N - 1);
       double2 Xrx, Xix, Xry, Xiy;
-      split2(pv[0][m], xb(0, km), Xrx, Xix);
-      split2(pv[1][m], xb(1, km), Xry, Xiy);
+      split2(pv[0][m], xb.pub(0, km), Xrx, Xix);
+      split2(pv[1][m], xb.pub(1, km), Xry, Xiy);
       const double kk = (double)k;
       kx_r[m] = mul_ik(Xrx, -kk);
       kx_i[m] = mul_ik(Xix, -kk);
@@ -160,7 +160,7 @@ __global__ __launch_bounds__(RP*(N / E)) void k_adj_row_qw(Dev d, AdjBufs b, dou
     const int k = t + T * m;
     if (k < KX) {
       double2 Xr, Xi;
-      split2(pv[2][m], xb(0, (N - k) & (N - 1)), Xr, Xi);
+      split2(pv[2][m], xb.pub(0, (N - k) & (N - 1)), Xr, Xi);
       b.A[2][ro + k] = cadd(kx_r[m], Xr);
       b.A[4][ro + k] = cadd(kx_i[m], Xi);
     }
@@ -342,8 +342,8 @@ __global__ __launch_bounds__(RP*(N / E)) void k_adj_row_nl(Dev d, AdjBufs b, int
     if (k < KX) {
       const int km = (N - k) & (N - 1);
       double2 Xu, Xv, Xz, Xq;
-      split2(pv[0][m], xb(0, km), Xu, Xv);
-      split2(pv[1][m], xb(1, km), Xz, Xq);
+      split2(pv[0][m], xb.pub(0, km), Xu, Xv);
+      split2(pv[1][m], xb.pub(1, km), Xz, Xq);
       const double kk = (double)k;
       const double2 a1 = mul_ik(Xv, -kk);
       b.A[0][ro + k] = Xu;
@@ -358,7 +358,7 @@ __global__ __launch_bounds__(RP*(N / E)) void k_adj_row_nl(Dev d, AdjBufs b, int
     const int k = t + T * m;
     if (k < KX) {
       double2 Xr, Xi;
-      split2(pv[2][m], xb(0, (N - k) & (N - 1)), Xr, Xi);
+      split2(pv[2][m], xb.pub(0, (N - k) & (N - 1)), Xr, Xi);
       b.A[4][ro + k] = Xr;
       b.A[5][ro + k] = Xi;
     }

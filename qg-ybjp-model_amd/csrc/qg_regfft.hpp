@@ -35,21 +35,23 @@ template <> struct RPlan<512, 16> { static constexpr int P = 3, R1 = 16, R2 = 8,
 template <> struct RPlan<512, 8>  { static constexpr int P = 3, R1 = 8,  R2 = 8, R3 = 8; };
 
 // LDS exchange buffers as seen by one thread.  gfx950 serves a b128 LDS read
-// 16 lanes per cycle over 64 four-byte banks (16 distinct 16-byte slots) and a
-// b128 write 8 lanes per cycle over 32 banks (8 slots); two lanes of one
-// group on the same slot at different addresses cost an extra cycle.
-//  - ST == 1 (row kernels): the buffers of one sequence group are contiguous,
-//    S buffers of SL elements with one pad element per E, so that the
-//    stride-E writes of the first pass fall on 8 distinct slots; rows are
-//    paired lane-wise (row_lane) and spaced row_xstride() apart, which is 8
-//    slots modulo 16, so that a 16-lane read of 8 consecutive positions of
-//    two rows is conflict-free.
+// in 4 cycles of 16 lanes (lane l in cycle parity(l>>2 ^ l>>3 ^ l>>4) +
+// 2*(l>>5)) over 64 four-byte banks, i.e. 16 distinct 16-byte slots, and a
+// b128 write in 8 cycles of 8 consecutive lanes over 8 slots; two lanes of
+// one cycle on the same slot at different addresses cost an extra cycle.
+//  - ST == 1 (row kernels): the buffers of one row are contiguous, S buffers
+//    of SL elements with one pad element per E for the transform passes, so
+//    that the stride-E writes of the first pass fall on distinct slots; the
+//    rows of a wave are row_xstride() apart (8 slots modulo 16) and lanes
+//    are assigned to (row, position) by row_lane, which together make every
+//    transform pass conflict-free.  The publish/read-back of whole sequences
+//    (pub) uses the same buffers without padding.
 //  - ST > 1 (column kernels, ST = TX columns): the TX columns of a workgroup
 //    are interleaved element by element with no padding: the 8 lanes of a
-//    write are 8 columns (distinct slots) and the 16 lanes of a read are 8
-//    columns at two consecutive positions (8 slots apart).
-// Every offset that is known at compile time stays additive, so LDS
-// instructions keep immediate offsets.
+//    write cycle are 8 columns (distinct slots) and a read cycle holds 8
+//    columns at two positions of opposite parity (8 slots apart).
+// Offsets known at compile time stay additive, so LDS instructions keep
+// immediate offsets.
 template <int N, int E, int ST>
 struct XBuf {
   static constexpr int SL = ST == 1 ? N + N / E : N;
@@ -58,14 +60,18 @@ struct XBuf {
     if constexpr (ST == 1) return p[s * SL + i + i / E];
     else return p[(s * SL + i) * ST];
   }
+  __device__ __forceinline__ double2& pub(int s, int i) const {
+    if constexpr (ST == 1) return p[s * SL + i];
+    else return p[(s * SL + i) * ST];
+  }
 };
 template <int N, int E>
 using RowBuf = XBuf<N, E, 1>;
 template <int N, int E>
 using ColBuf = XBuf<N, E, ((64 / (N / E)) > 8 ? 64 / (N / E) : 8)>;
 
-// Elements of LDS a row workgroup with RP rows needs for SB buffers per row,
-// and the spacing of consecutive rows (8 slots modulo 16).
+// Elements of LDS a row workgroup needs per row for SB buffers, i.e. the
+// spacing of consecutive rows (8 slots modulo 16).
 template <int N, int E, int SB>
 constexpr int row_xstride() {
   constexpr int L = SB * RowBuf<N, E>::SL;
@@ -76,20 +82,33 @@ template <int N, int E, int SB, int TX>
 constexpr int col_xlen() { return TX * SB * N; }
 
 // Lane -> (position t, row p) of a row workgroup of RP rows with T threads
-// per row.  Lanes go in chunks of 8 consecutive positions; chunks of rows 2a
-// and 2a+1 alternate, so each 16-lane group reads 8 positions of two rows and
-// each 8-lane group stays inside one row (one 128-byte line of global
-// memory).  Odd RP or T < 8 keep the plain mapping.
+// per row.  Every quad of lanes holds 4 consecutive positions of one row (64
+// contiguous bytes of a global row); for T = 8, 16, 32 and whole waves the
+// 16 quads of a wave take the (row, position-quad) cells of a table chosen so
+// that, with the row spacing above, no LDS cycle of the transform passes
+// (first-pass writes, middle and last-pass reads, middle-pass writes) or of
+// the unpadded publish/read-back has two lanes on one slot.  The tables
+// are packed 6 bits per quad: row << 3 | position-quad.  Other shapes keep
+// the plain mapping.
+template <int T>
+struct RowQuads {
+  static constexpr unsigned long long lo = 0, hi = 0;
+};
+template <> struct RowQuads<8> { static constexpr unsigned long long lo = 0x670621c68829ull, hi = 0x211009e01439ull; };
+template <> struct RowQuads<16> { static constexpr unsigned long long lo = 0x2422910c801bull, hi = 0x60b4814d065aull; };
+template <> struct RowQuads<32> { static constexpr unsigned long long lo = 0x1ca2ce0c63c2ull, hi = 0x30900d205101ull; };
+
 template <int T, int RP>
 __device__ __forceinline__ void row_lane(int tid, int& t, int& p) {
-  if constexpr (RP % 2 != 0 || T < 8) {
+  if constexpr ((T == 8 || T == 16 || T == 32) && (RP * T) % 64 == 0) {
+    const int w = tid >> 6, l = tid & 63, q = l >> 2;
+    const unsigned long long k = q < 8 ? RowQuads<T>::lo : RowQuads<T>::hi;
+    const int e = (int)((k >> (6 * (q & 7))) & 63);
+    t = 4 * (e & 7) + (l & 3);
+    p = (64 / T) * w + (e >> 3);
+  } else {
     t = tid % T;
     p = tid / T;
-  } else {
-    constexpr int C = T / 8;
-    const int ch = tid >> 3;
-    t = (tid & 7) + 8 * ((ch >> 1) % C);
-    p = 2 * ((ch >> 1) / C) + (ch & 1);
   }
 }
 
@@ -317,21 +336,21 @@ __device__ __forceinline__ void rf_publish_range(const double2 (&v)[S][E], const
 #pragma unroll
   for (int s = 0; s < G; ++s)
 #pragma unroll
-    for (int m = 0; m < E; ++m) xb(s, t + T * m) = v[S0 + s][m];
+    for (int m = 0; m < E; ++m) xb.pub(s, t + T * m) = v[S0 + s][m];
   __syncthreads();
 }
 
 // Exchange the strided distribution through the buffer so that every thread
 // can read arbitrary positions of the sequence (used for the k <-> N-k pairs
 // of two-for-one real transforms and for the realit fill).  After the call,
-// xb(0, i) holds element i; the caller synchronises before the buffer is
-// written again.
+// xb.pub(0, i) holds element i; the caller synchronises before the buffer
+// is written again.
 template <int N, int E, int ST>
 __device__ __forceinline__ void rf_publish(const double2 (&v)[E], const XBuf<N, E, ST>& xb, int t) {
   constexpr int T = N / E;
   __syncthreads();
 #pragma unroll
-  for (int m = 0; m < E; ++m) xb(0, t + T * m) = v[m];
+  for (int m = 0; m < E; ++m) xb.pub(0, t + T * m) = v[m];
   __syncthreads();
 }
 

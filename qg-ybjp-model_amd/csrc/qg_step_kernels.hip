@@ -74,7 +74,7 @@ __global__ __launch_bounds__((SH<N, E>::COLT)) void k_ps_inv(Geo g, const double
 #pragma unroll
         for (int m = 0; m < E; ++m) {
           const int iky = t + T * m;
-          if (iky > N / 2) v[m] = cconj(xb(0, N - iky));
+          if (iky > N / 2) v[m] = cconj(xb.pub(0, N - iky));
         }
       }
     }
@@ -163,8 +163,8 @@ __global__ __launch_bounds__(RP*(N / E)) void k_row_nl(Dev d, int Win, double sc
     if (k < KX) {
       const int km = (N - k) & (N - 1);
       double2 FzR, FzI, FuR, FvR;
-      split2(pv[0][m], xb(0, km), FzR, FzI);
-      split2(pv[1][m], xb(1, km), FuR, FvR);
+      split2(pv[0][m], xb.pub(0, km), FzR, FzI);
+      split2(pv[1][m], xb.pub(1, km), FuR, FvR);
       double2 w2 = mul_ik(FuR, (double)k), w3 = FvR;
       if (lin) { w2 = zero2(); w3 = zero2(); }
       if (!pas) w2 = make_double2(w2.x - 0.5 * FzI.x, w2.y - 0.5 * FzI.y);
@@ -182,8 +182,8 @@ __global__ __launch_bounds__(RP*(N / E)) void k_row_nl(Dev d, int Win, double sc
     if (k < KX) {
       const int km = (N - k) & (N - 1);
       double2 FuI, FvI, Fuq, Fvq;
-      split2(pv[2][m], xb(0, km), FuI, FvI);
-      split2(pv[3][m], xb(1, km), Fuq, Fvq);
+      split2(pv[2][m], xb.pub(0, km), FuI, FvI);
+      split2(pv[3][m], xb.pub(1, km), Fuq, Fvq);
       const double kk = (double)k;
       double2 w0 = mul_ik(Fuq, kk), w1 = Fvq, w4 = mul_ik(FuI, kk), w5 = FvI;
       if (lin) { w0 = zero2(); w1 = zero2(); w4 = zero2(); w5 = zero2(); }
@@ -296,8 +296,8 @@ __global__ __launch_bounds__((SH<N, E>::COLT)) void k_col_upd(Dev d, int euler, 
         for (int m = 0; m < E; ++m) {
           const int iky = t + T * m;
           if (iky > N / 2) {
-            v[0][m] = cconj(xb(0, N - iky));
-            v[1][m] = cconj(xb(1, N - iky));
+            v[0][m] = cconj(xb.pub(0, N - iky));
+            v[1][m] = cconj(xb.pub(1, N - iky));
           }
         }
       }
@@ -307,7 +307,7 @@ __global__ __launch_bounds__((SH<N, E>::COLT)) void k_col_upd(Dev d, int euler, 
 #pragma unroll
         for (int m = 0; m < E; ++m) {
           const int iky = t + T * m;
-          if (iky > N / 2) v[0][m] = cconj(xb(0, N - iky));
+          if (iky > N / 2) v[0][m] = cconj(xb.pub(0, N - iky));
         }
       }
       rf_publish_range<N, E, 2, 1, 1>(v, xb, t);
@@ -315,7 +315,7 @@ __global__ __launch_bounds__((SH<N, E>::COLT)) void k_col_upd(Dev d, int euler, 
 #pragma unroll
         for (int m = 0; m < E; ++m) {
           const int iky = t + T * m;
-          if (iky > N / 2) v[1][m] = cconj(xb(0, N - iky));
+          if (iky > N / 2) v[1][m] = cconj(xb.pub(0, N - iky));
         }
       }
     }
@@ -375,7 +375,7 @@ __global__ __launch_bounds__(RP*(N / E)) void k_row_qw(Dev d, double scale) {
     const int k = t + T * m;
     if (k < KX) {
       double2 FJ, FM;
-      split2(pv[0][m], xb(0, (N - k) & (N - 1)), FJ, FM);
+      split2(pv[0][m], xb.pub(0, (N - k) & (N - 1)), FJ, FM);
       const double c4 = 0.25 * (double)k * (double)k;
       I[3][ro + k] = cscale(make_double2(FJ.x - c4 * FM.x, FJ.y - c4 * FM.y), scale);
       I[5][ro + k] = cscale(FM, scale);
