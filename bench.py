@@ -153,6 +153,12 @@ def adjoint_bench(steps: int = 200, every: int = 20, repeats: int = 2):
     psi = ic.eddy_psi(128, 64, 4, 3, Bu=P.Bu)
     BR, BI = ic.wave_b(128, 64)
     m = qgybj.Member(P)
+    # kernel variants tuned for this grid (the forward and the gradient runs
+    # both use them)
+    m.set_state_physical(psi, BR, BI)
+    m.step(3)
+    variants, _ = m.tune(5)
+    m.set_variants(variants)
     adj = qgybj.Adjoint(m, checkpoint_every=every)
     adj.objective(psi, BR, BI, 2)  # warm-up
     t_f, t_g = [], []
@@ -189,7 +195,8 @@ def adjoint_bench(steps: int = 200, every: int = 20, repeats: int = 2):
     return {"workload": f"128^2x64 FP64, exponential N^2, {steps} steps, checkpoints every {every}, "
                         "objective: deep wave energy, control: initial psi",
             "forward_s": round(tf, 4), "gradient_s": round(tg, 4), "adjoint_over_forward": round(tg / tf, 3),
-            "J": J, "fd_max_rel_err_8_directions": max(fd_err), "dot_product_rel_err": abs(lhs - rhs) / abs(lhs)}
+            "J": J, "fd_max_rel_err_8_directions": max(fd_err), "dot_product_rel_err": abs(lhs - rhs) / abs(lhs),
+            "variants": [int(x) for x in variants]}
 
 
 def large_bench(steps: int = 500, every: int = 50, batch_steps: int = 200, batch_every: int = 20):

@@ -61,7 +61,8 @@ struct qg_adj {
   double2* x[NSTATE] = {};
   double2* s[12] = {};
   double* p[24] = {};
-  double2* F[6] = {};  // fused adjoint: forward (kx, y) planes
+  double2* F[6] = {};  // fused adjoint: forward (kx, y) planes of step n
+  double2* F1[6] = {};  // and of B^{n+1} (entries 2..5: BR', ky BR', BI', ky BI')
   double2* A[6] = {};  // fused adjoint: adjoint (kx, y) planes
   int fused = 1;       // 1: fused adjoint kernels, 0: generic kernels
   double* part = nullptr;
@@ -473,19 +474,30 @@ int adj_step_fused(qg_adj* a, const double2* const* fn, const double2* BR1, cons
     dd.AI = s[7];
     AHIP(a, qg::launch_solve(dd, a->st, 0, 2, x[F_PSI]));
   }
+  // every forward (kx, y) plane the step needs, in one launch: those of step
+  // n as row_nl reads them (the Euler step's fields are not yet
+  // box-restricted) and those of B^{n+1} for the feedback term.  q, BR, BI
+  // are adjacent in the segment slot, and so are F0, F2, F4; likewise BR',
+  // BI' and F1[2], F1[4] and F1[3], F1[5].
+  const int Win = euler ? g.iktx : g.KX;
+  {
+    qg::ColInvJobs jobs{};
+    jobs.j[0] = qg::ColInvJob{fn[0], {a->F[1], a->F[3], a->F[5]}, g.KX, 1, 1, g.nz};
+    jobs.j[1] = qg::ColInvJob{fn[1], {a->F[0], nullptr, nullptr}, Win, euler ? 0 : 1, 0, 3 * g.nz};
+    jobs.n = 2;
+    if (k.feedback) {
+      jobs.j[2] = qg::ColInvJob{BR1, {a->F1[2], a->F1[3], nullptr}, g.KX, 1, 1, 2 * g.nz};
+      jobs.n = 3;
+    }
+    AHIP(a, qg::launch_col_inv(d, a->st, jobs));
+  }
   if (k.feedback) {
     AHIP(a, qg::launch_adj_col_qw(d, a->st, b));
-    // BR', BI' are adjacent in the segment slot, and so are F2, F4 and F3, F5
-    AHIP(a, qg::launch_col_inv(d, a->st, BR1, a->F[2], a->F[3], nullptr, g.KX, 1, 1, 2 * g.nz));
-    AHIP(a, qg::launch_adj_row_qw(d, a->st, b));
+    qg::AdjBufs b1 = b;
+    for (int i = 2; i < 6; ++i) b1.F[i] = a->F1[i];
+    AHIP(a, qg::launch_adj_row_qw(d, a->st, b1));
   }
   AHIP(a, qg::launch_adj_col_upd(d, a->st, b, euler ? 1 : 0, k.feedback ? 1 : 0));
-  // forward (kx, y) planes of step n as row_nl reads them (the Euler step's
-  // fields are not yet box-restricted)
-  const int Win = euler ? g.iktx : g.KX;
-  AHIP(a, qg::launch_col_inv(d, a->st, fn[0], a->F[1], a->F[3], a->F[5], g.KX, 1, 1, g.nz));
-  // q, BR, BI are adjacent in the segment slot, and so are F0, F2, F4
-  AHIP(a, qg::launch_col_inv(d, a->st, fn[1], a->F[0], nullptr, nullptr, Win, euler ? 0 : 1, 0, 3 * g.nz));
   AHIP(a, qg::launch_adj_row_nl(d, a->st, b, Win, euler ? 1 : 0));
   AHIP(a, qg::launch_adj_col_out(d, a->st, b));
   return QG_OK;
@@ -717,13 +729,16 @@ int qg_adj_create(qg_ctx* c, int32_t checkpoint_every, double z_deep, qg_adj** o
   for (int i = 0; i < NSTATE && rc == QG_OK; ++i) rc = aalloc(a, &a->x[i], a->S);
   for (int i = 0; i < 12 && rc == QG_OK; ++i) rc = aalloc(a, &a->s[i], a->S);
   rc = aalloc(a, &a->p[12], a->NP);  // control transfers; the rest on first use
-  // forward planes in the order F0, F2, F4, F1, F3, F5 so that the y-inverses
-  // of (q, BR, BI) and of (BR', BI') run as single launches over stacked levels
+  // forward planes in the order F0, F2, F4, F1, F3, F5 and then F1[2],
+  // F1[4], F1[3], F1[5], so that the y-inverses of (q, BR, BI) and of
+  // (BR', BI') are jobs over stacked levels
   double2* fblk = nullptr;
-  if (rc == QG_OK) rc = aalloc(a, &fblk, 6 * a->S);
+  if (rc == QG_OK) rc = aalloc(a, &fblk, 10 * a->S);
   if (rc == QG_OK) {
     const int order[6] = {0, 3, 1, 4, 2, 5};
     for (int i = 0; i < 6; ++i) a->F[i] = fblk + (size_t)order[i] * a->S;
+    const int order1[4] = {6, 8, 7, 9};
+    for (int i = 0; i < 4; ++i) a->F1[2 + i] = fblk + (size_t)order1[i] * a->S;
   }
   for (int i = 0; i < 6 && rc == QG_OK; ++i) rc = aalloc(a, &a->A[i], a->S);
   double2* tq = nullptr;

@@ -60,19 +60,20 @@ __global__ __launch_bounds__((SH<N, E>::COLT)) void k_adj_col_qw(Dev d, AdjBufs 
   const ColBuf<N, E> xb{lds + c};
   const bool colok = kx < g.KX;
   const size_t sz = (size_t)z * g.ikty * g.sp, pz = (size_t)z * g.n2 * g.sp;
+  // loads first, unconditionally (column 0 stands in for columns beyond KX),
+  // then the mask: a zero-initialised branch around each load would make
+  // the compiler wait on it
   double2 v[2][E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) v[0][m] = b.qstar[sz + (size_t)(t + T * m) * g.sp + (colok ? kx : 0)];
 #pragma unroll
   for (int m = 0; m < E; ++m) {
     const int iky = t + T * m;
-    double2 a = zero2();
-    double ky2 = 0.0;
-    if (colok && lmask(kx, iky, g)) {
-      const int ky = ky_of(iky, g.n2);
-      a = cscale(b.qstar[sz + (size_t)iky * g.sp + kx], 1.0 / hweight(kx, ky));
-      ky2 = (double)ky * (double)ky;
-    }
+    const int ky = ky_of(iky, g.n2);
+    const bool on = colok && lmask(kx, iky, g);
+    const double2 a = on ? cscale(v[0][m], 1.0 / hweight(kx, ky)) : zero2();
     v[0][m] = a;
-    v[1][m] = cscale(a, ky2);
+    v[1][m] = cscale(a, (double)ky * (double)ky);
   }
   if (kx0 == 0) realit_col0<N, E, 2, 0, 2>(v, xb, t, c);
   reg_fft_multi<N, E, 2, 2, true>(v, xb, t, d.tw);
@@ -106,16 +107,25 @@ __global__ __launch_bounds__(RP*(N / E)) void k_adj_row_qw(Dev d, AdjBufs b, dou
   const RowBuf<N, E> xb{lds + p * RS};
   const size_t ro = ((size_t)blockIdx.x * RP + p) * g.sp;
   const int KX = g.KX;
+  // loads first, unconditionally (entry 0 stands in for k >= KX), then the
+  // mask
+  double2 in[6][E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int pos = t + T * m;
+    const int k = pos <= N / 2 ? pos : N - pos;
+    const size_t a = ro + (k < KX ? k : 0);
+    in[0][m] = b.F[2][a]; in[1][m] = b.F[3][a]; in[2][m] = b.F[4][a];
+    in[3][m] = b.F[5][a]; in[4][m] = b.A[0][a]; in[5][m] = b.A[1][a];
+  }
   double2 z[4][E];
 #pragma unroll
   for (int m = 0; m < E; ++m) {
     const int pos = t + T * m;
     const int k = pos <= N / 2 ? pos : N - pos;
-    double2 br = zero2(), bry = zero2(), bi = zero2(), biy = zero2(), aj = zero2(), ajy = zero2();
-    if (k < KX) {
-      br = b.F[2][ro + k]; bry = b.F[3][ro + k]; bi = b.F[4][ro + k]; biy = b.F[5][ro + k];
-      aj = b.A[0][ro + k]; ajy = b.A[1][ro + k];
-    }
+    const bool ok = k < KX;
+    const double2 br = ok ? in[0][m] : zero2(), bry = ok ? in[1][m] : zero2(), bi = ok ? in[2][m] : zero2();
+    const double2 biy = ok ? in[3][m] : zero2(), aj = ok ? in[4][m] : zero2(), ajy = ok ? in[5][m] : zero2();
     const double kk = (double)k;
     z[0][m] = pack2<N>(br, bi, pos);
     z[1][m] = pack2<N>(mul_ik(br, kk), mul_ik(bi, kk), pos);
@@ -296,22 +306,25 @@ __global__ __launch_bounds__(RP*(N / E)) void k_adj_row_nl(Dev d, AdjBufs b, int
     const int pos = t + T * m;
     const int k = pos <= N / 2 ? pos : N - pos;
     const double kk = (double)k, k2 = kk * kk;
-    double2 ps = zero2(), py = zero2(), pyy = zero2(), pq = zero2(), pbr = zero2(), pbi = zero2();
-    if (k < Win) {
-      ps = b.F[1][ro + k]; py = b.F[3][ro + k]; pyy = b.F[5][ro + k];
-      pq = b.F[0][ro + k]; pbr = b.F[2][ro + k]; pbi = b.F[4][ro + k];
-    }
+    // unconditional loads (entry 0 stands in for masked positions), then the
+    // mask: a zero-initialised branch around each load would make the
+    // compiler wait on it
+    const size_t af = ro + (k < Win ? k : 0), aa = ro + (k < KX ? k : 0);
+    const double2 lps = b.F[1][af], lpy = b.F[3][af], lpyy = b.F[5][af];
+    const double2 lpq = b.F[0][af], lpbr = b.F[2][af], lpbi = b.F[4][af];
+    const double2 laq = b.A[0][aa], laqy = b.A[1][aa], lar = b.A[2][aa];
+    const double2 lary = b.A[3][aa], lai = b.A[4][aa], laiy = b.A[5][aa];
+    const bool okf = k < Win, oka = k < KX;
+    const double2 ps = okf ? lps : zero2(), py = okf ? lpy : zero2(), pyy = okf ? lpyy : zero2();
+    const double2 pq = okf ? lpq : zero2(), pbr = okf ? lpbr : zero2(), pbi = okf ? lpbi : zero2();
     const double2 U = make_double2(-py.x, -py.y);
     const double2 V = mul_ik(ps, kk);
     const double2 Z = make_double2(-k2 * ps.x + pyy.x, -k2 * ps.y + pyy.y);
     z[0][m] = pack2<N>(U, V, pos);
     z[1][m] = pack2<N>(Z, pq, pos);
     z[2][m] = pack2<N>(pbr, pbi, pos);
-    double2 aq = zero2(), aqy = zero2(), ar = zero2(), ary = zero2(), ai = zero2(), aiy = zero2();
-    if (k < KX) {
-      aq = b.A[0][ro + k]; aqy = b.A[1][ro + k]; ar = b.A[2][ro + k];
-      ary = b.A[3][ro + k]; ai = b.A[4][ro + k]; aiy = b.A[5][ro + k];
-    }
+    const double2 aq = oka ? laq : zero2(), aqy = oka ? laqy : zero2(), ar = oka ? lar : zero2();
+    const double2 ary = oka ? lary : zero2(), ai = oka ? lai : zero2(), aiy = oka ? laiy : zero2();
     z[3][m] = pack2<N>(mul_ik(aq, -s1 * kk), cscale(aqy, s1), pos);
     z[4][m] = pack2<N>(mul_ik(ar, -s1 * kk), cscale(ary, s1), pos);
     z[5][m] = pack2<N>(mul_ik(ai, -s1 * kk), cscale(aiy, s1), pos);

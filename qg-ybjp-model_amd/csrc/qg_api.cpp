@@ -171,12 +171,18 @@ int step_once(qg_ctx* c) {
   const int box = full ? 0 : 1;
   // u, v, zeta ingredients from psi (compute_velo derivatives.f90:1302-1311,
   // refraction_waqg :831-845)
-  KRUN(c, qg::K_COL_INV, qg::launch_col_inv(d, st, d.psi, d.I[1], d.I[3], d.I[5], W, box, 1, g.nz));
-  if (full) {
-    // unmasked fields as the state holds them (the initial B is un-dealiased)
-    KRUN(c, qg::K_COL_INV, qg::launch_col_inv(d, st, d.q, d.I[0], nullptr, nullptr, W, 0, 0, g.nz));
-    KRUN(c, qg::K_COL_INV, qg::launch_col_inv(d, st, d.BR, d.I[2], nullptr, nullptr, W, 0, 0, g.nz));
-    KRUN(c, qg::K_COL_INV, qg::launch_col_inv(d, st, d.BI, d.I[4], nullptr, nullptr, W, 0, 0, g.nz));
+  {
+    qg::ColInvJobs jobs{};
+    jobs.j[0] = qg::ColInvJob{d.psi, {d.I[1], d.I[3], d.I[5]}, W, box, 1, g.nz};
+    jobs.n = 1;
+    if (full) {
+      // unmasked fields as the state holds them (the initial B is un-dealiased)
+      jobs.j[1] = qg::ColInvJob{d.q, {d.I[0], nullptr, nullptr}, W, 0, 0, g.nz};
+      jobs.j[2] = qg::ColInvJob{d.BR, {d.I[2], nullptr, nullptr}, W, 0, 0, g.nz};
+      jobs.j[3] = qg::ColInvJob{d.BI, {d.I[4], nullptr, nullptr}, W, 0, 0, g.nz};
+      jobs.n = 4;
+    }
+    KRUN(c, qg::K_COL_INV, qg::launch_col_inv(d, st, jobs));
   }
   KRUN(c, qg::K_ROW_NL, qg::launch_row_nl(d, st, W));
   // the update reads A^n and overwrites B in place: wait for the previous

@@ -103,6 +103,19 @@ const char* kernel_name(int id);
 // W columns; box rows only if box; mask applies L
 hipError_t launch_col_inv(const Dev& d, hipStream_t st, const double2* F, double2* o0, double2* o1, double2* o2,
                           int W, int box, int mask, int nz);
+// several independent inverses in one launch (one job per input field; a
+// job's nz planes are consecutive in the input and in each output)
+struct ColInvJob {
+  const double2* F;
+  double2* o[3];
+  int W, box, mask, nz;
+};
+struct ColInvJobs {
+  static constexpr int MAXJ = 4;
+  ColInvJob j[MAXJ];
+  int n;
+};
+hipError_t launch_col_inv(const Dev& d, hipStream_t st, const ColInvJobs& jobs);
 hipError_t launch_row_nl(const Dev& d, hipStream_t st, int Win);
 hipError_t launch_col_upd(const Dev& d, hipStream_t st, int euler, int has_dq);
 hipError_t launch_row_qw(const Dev& d, hipStream_t st);

@@ -37,30 +37,34 @@ namespace {
 // from these in row_nl: compute_velo derivatives.f90:1302-1311,
 // refraction_waqg :831-845), the mask L and the reference's realit fill of
 // (0,-ky) after the multiplier (fft.f90:71-107).  Columns kx < W; rows of the
-// dealiasing box only if box.
+// dealiasing box only if box.  One launch runs up to four independent jobs
+// (ColInvJobs): grid rows are the jobs' planes one after the other.
 // ---------------------------------------------------------------------------
 template <int N, int E>
-__global__ __launch_bounds__((SH<N, E>::COLT)) void k_ps_inv(Geo g, const double2* __restrict__ tw,
-                                                                   const double2* __restrict__ F, double2* o0,
-                                                                   double2* o1, double2* o2, int W, int box,
-                                                                   int mask) {
+__global__ __launch_bounds__((SH<N, E>::COLT)) void k_ps_inv(Geo g, const double2* __restrict__ tw, ColInvJobs jobs) {
   constexpr int T = SH<N, E>::T, TX = SH<N, E>::TX;
   __shared__ double2 lds[col_xlen<N, E, 1, TX>()];
+  int z = blockIdx.y, ji = 0;
+  while (ji + 1 < jobs.n && z >= jobs.j[ji].nz) z -= jobs.j[ji++].nz;
+  const ColInvJob& J = jobs.j[ji];
+  const int W = J.W;
   const int c = threadIdx.x % TX, t = threadIdx.x / TX;
-  const int kx0 = blockIdx.x * TX, kx = kx0 + c, z = blockIdx.y;
+  const int kx0 = blockIdx.x * TX, kx = kx0 + c;
+  if (kx0 >= W) return;  // the grid is sized for the widest job
   const ColBuf<N, E> xb{lds + c};
-  const double2* Fz = F + (size_t)z * g.ikty * g.sp;
+  const double2* Fz = J.F + (size_t)z * g.ikty * g.sp;
   double2 v0[E];
 #pragma unroll
   for (int m = 0; m < E; ++m) {
     const int iky = t + T * m;
-    v0[m] = (kx < W && (!box || in_box_row(iky, g)) && (!mask || lmask(kx, iky, g))) ? Fz[(size_t)iky * g.sp + kx]
-                                                                                      : zero2();
+    v0[m] = (kx < W && (!J.box || in_box_row(iky, g)) && (!J.mask || lmask(kx, iky, g)))
+                ? Fz[(size_t)iky * g.sp + kx]
+                : zero2();
   }
   const size_t pz = (size_t)z * g.n2 * g.sp;
 #pragma unroll 1
   for (int o = 0; o < 3; ++o) {
-    double2* out = o == 0 ? o0 : (o == 1 ? o1 : o2);
+    double2* out = J.o[o];
     if (!out) break;
     double2 v[E];
 #pragma unroll
@@ -432,11 +436,15 @@ __global__ __launch_bounds__((SH<N, E>::COLT)) void k_col_qw(Dev d) {
 template <int N>
 struct LS {
   template <int E>
-  static hipError_t ps_inv_e(const Dev& d, hipStream_t st, const double2* F, double2* o0, double2* o1, double2* o2,
-                             int W, int box, int mask, int nz) {
+  static hipError_t ps_inv_e(const Dev& d, hipStream_t st, const ColInvJobs& jobs) {
     using S = SH<N, E>;
+    int W = 0, nz = 0;
+    for (int i = 0; i < jobs.n; ++i) {
+      W = jobs.j[i].W > W ? jobs.j[i].W : W;
+      nz += jobs.j[i].nz;
+    }
     dim3 grid((W + S::TX - 1) / S::TX, nz);
-    hipLaunchKernelGGL((k_ps_inv<N, E>), grid, dim3(S::COLT), 0, st, d.g, d.tw, F, o0, o1, o2, W, box, mask);
+    hipLaunchKernelGGL((k_ps_inv<N, E>), grid, dim3(S::COLT), 0, st, d.g, d.tw, jobs);
     return hipGetLastError();
   }
   template <int E, int WG>
@@ -467,10 +475,8 @@ struct LS {
     hipLaunchKernelGGL((k_col_qw<N, E, SB>), grid, dim3(S::COLT), 0, st, d);
     return hipGetLastError();
   }
-  static hipError_t ps_inv(const Dev& d, hipStream_t st, const double2* F, double2* o0, double2* o1, double2* o2,
-                           int W, int box, int mask, int nz) {
-    return (d.tune.e16[0] & 1) ? ps_inv_e<16>(d, st, F, o0, o1, o2, W, box, mask, nz)
-                         : ps_inv_e<8>(d, st, F, o0, o1, o2, W, box, mask, nz);
+  static hipError_t ps_inv(const Dev& d, hipStream_t st, const ColInvJobs& jobs) {
+    return (d.tune.e16[0] & 1) ? ps_inv_e<16>(d, st, jobs) : ps_inv_e<8>(d, st, jobs);
   }
   // row variants: 0 = E8 in 64-thread blocks, 1 = E16/64, 2 = E8/128,
   // 3 = E16/32 (rows per block = threads / (N/E), at least one)
@@ -523,9 +529,16 @@ struct LS {
     default: return hipErrorInvalidValue;    \
   }
 
+hipError_t launch_col_inv(const Dev& d, hipStream_t st, const ColInvJobs& jobs) {
+  if (jobs.n < 1 || jobs.n > ColInvJobs::MAXJ) return hipErrorInvalidValue;
+  QG_SDISPATCH(d.g.n2, ps_inv(d, st, jobs))
+}
 hipError_t launch_col_inv(const Dev& d, hipStream_t st, const double2* F, double2* o0, double2* o1, double2* o2,
                           int W, int box, int mask, int nz) {
-  QG_SDISPATCH(d.g.n2, ps_inv(d, st, F, o0, o1, o2, W, box, mask, nz))
+  ColInvJobs jobs{};
+  jobs.j[0] = ColInvJob{F, {o0, o1, o2}, W, box, mask, nz};
+  jobs.n = 1;
+  return launch_col_inv(d, st, jobs);
 }
 hipError_t launch_row_nl(const Dev& d, hipStream_t st, int Win) { QG_SDISPATCH(d.g.n1, row_nl(d, st, Win)) }
 hipError_t launch_col_upd(const Dev& d, hipStream_t st, int euler, int has_dq) {
