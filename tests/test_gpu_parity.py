@@ -200,7 +200,7 @@ def test_vertical_partitions(n3):
     m.close()
 
 
-@pytest.mark.parametrize("n,n3", [(32, 4), (64, 8), (256, 4), (512, 2)])
+@pytest.mark.parametrize("n,n3", [(32, 4), (64, 8), (128, 4), (256, 4), (512, 2)])
 @pytest.mark.parametrize("variant", [(0, 0, 0, 0, 0), (1, 1, 1, 1, 1), (0, 2, 0, 2, 0), (1, 3, 1, 3, 1),
                                      (0, 0, 2, 0, 2), (1, 1, 3, 1, 3)])
 def test_kernel_variants(n, n3, variant):
