@@ -102,6 +102,11 @@ program qg_driver
     call read_field(trim(ic_prefix)//'BR.in', BRr)
     call read_field(trim(ic_prefix)//'BI.in', BIr)
   end if
+  if (grad_steps > 0) then
+    call gradient_mode()
+    rc = qg_destroy(ctx)
+    stop
+  end if
   call check(qg_set_state_physical(ctx, c_loc(psir), c_loc(BRr), c_loc(BIr), int(n1, c_int32_t)), 'set_state')
   deallocate(psir, BRr, BIr)
 
@@ -143,6 +148,37 @@ program qg_driver
   rc = qg_destroy(ctx)
 
 contains
+
+  ! J (deep wave energy after grad_steps steps) and dJ/dpsi0 through the
+  ! adjoint entry points, from the initial conditions in psir, BRr, BIr
+  subroutine gradient_mode()
+    type(c_ptr) :: adj
+    real(c_double), target :: J
+    real(c_double), allocatable, target :: g(:, :, :)
+    integer :: v, e
+    e = qg_adj_create(ctx, int(grad_every, c_int32_t), grad_zdeep, adj)
+    if (e /= QG_OK) then
+      write(*, '(a,i0,a,a)') 'qg_driver: qg_adj_create failed, code ', e, ': ', trim(qg_error_string(ctx))
+      stop 1
+    end if
+    allocate(g(n1, n1, n3))
+    e = qg_adj_gradient(adj, c_loc(psir), c_loc(BRr), c_loc(BIr), int(n1, c_int32_t), int(grad_steps, c_int32_t), &
+                        c_loc(J), c_loc(g))
+    if (e /= QG_OK) then
+      write(*, '(a,i0,a,a)') 'qg_driver: qg_adj_gradient failed, code ', e, ': ', trim(c_string(qg_adj_error(adj)))
+      stop 1
+    end if
+    write(*, '(a,i0,a,es25.17)') '# grad steps ', grad_steps, ' J ', J
+    write(*, '(a,es25.17)') '# grad norm ', sqrt(sum(g*g))
+    if (len_trim(out_prefix) > 0) then
+      open(newunit=v, file=trim(out_prefix)//'grad.bin', access='stream', form='unformatted', &
+           status='replace', action='write')
+      write(v) g
+      close(v)
+    end if
+    e = qg_adj_destroy(adj)
+    deallocate(g)
+  end subroutine gradient_mode
 
   subroutine check(code, what)
     integer(c_int), intent(in) :: code

@@ -29,11 +29,17 @@ module qg_run_params
   integer :: out_every = 0             ! dump psi and L+A in r-space every out_every steps (0: never)
   character(len=256) :: ic_prefix = ' '   ! read <prefix>psi.in, BR.in, BI.in (float64 n1*n1*n3); blank: analytic
   character(len=256) :: out_prefix = ' '  ! final spectral state to <prefix>state.bin; blank: none
+  ! gradient mode (no counterpart in the reference): grad_steps > 0 computes, instead of the run,
+  ! the deep wave energy after grad_steps steps and its gradient with respect to the initial psi
+  ! (checkpoints every grad_every steps, levels with z_s < grad_zdeep); the gradient goes to
+  ! <out_prefix>grad.bin (float64 n1*n1*n3)
+  integer :: grad_steps = 0, grad_every = 20
+  double precision :: grad_zdeep = 0.8d0*6.28318530717958647692d0
 
   namelist /qg/ n1, n3, delt, gamma, Ro, Bu, W2F, coeff1, coeff2, coeff1w, coeff2w, coeffz, &
                 ilap1, ilap2, ilap1w, ilap2w, fixed_flow, no_feedback, passive_scalar, linear, &
                 no_dispersion, inviscid, stratification, n2_ratio, itermax, diag_every, out_every, &
-                ic_prefix, out_prefix
+                ic_prefix, out_prefix, grad_steps, grad_every, grad_zdeep
 
 contains
 

@@ -87,25 +87,153 @@ module qgybj_c
       type(c_ptr), value :: ctx, r, k
       integer(c_int32_t), value :: ldx, nz
     end function
+
+    integer(c_int) function qg_sync(ctx) bind(C, name="qg_sync")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
+    ! batches: `members` members with the same grid and parameters in one
+    ! context; every host array gains a trailing member dimension
+    integer(c_int) function qg_create_batch(p, members, ctx) bind(C, name="qg_create_batch")
+      import :: c_int, c_int32_t, c_ptr, qg_params_t
+      type(qg_params_t), intent(in) :: p
+      integer(c_int32_t), value :: members
+      type(c_ptr), intent(out) :: ctx
+    end function
+
+    integer(c_int) function qg_members(ctx, members) bind(C, name="qg_members")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), intent(out) :: members
+    end function
+
+    ! kernel variants (QG_NVARIANTS values) and the tuner
+    integer(c_int) function qg_set_variants(ctx, v, n) bind(C, name="qg_set_variants")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), intent(in) :: v(*)
+      integer(c_int32_t), value :: n
+    end function
+
+    integer(c_int) function qg_get_variants(ctx, v, n) bind(C, name="qg_get_variants")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), intent(out) :: v(*)
+      integer(c_int32_t), value :: n
+    end function
+
+    integer(c_int) function qg_tune(ctx, steps, best, best_ms) bind(C, name="qg_tune")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx, best, best_ms
+      integer(c_int32_t), value :: steps
+    end function
+
+    ! ensembles of contexts advanced together (graph replays)
+    integer(c_int) function qg_ensemble_create(members, n, ens) bind(C, name="qg_ensemble_create")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), intent(in) :: members(*)
+      integer(c_int32_t), value :: n
+      type(c_ptr), intent(out) :: ens
+    end function
+
+    integer(c_int) function qg_ensemble_destroy(ens) bind(C, name="qg_ensemble_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ens
+    end function
+
+    type(c_ptr) function qg_ensemble_error(ens) bind(C, name="qg_ensemble_error")
+      import :: c_ptr
+      type(c_ptr), value :: ens
+    end function
+
+    integer(c_int) function qg_ensemble_set_graph_steps(ens, k) bind(C, name="qg_ensemble_set_graph_steps")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ens
+      integer(c_int32_t), value :: k
+    end function
+
+    integer(c_int) function qg_ensemble_step(ens, nsteps) bind(C, name="qg_ensemble_step")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ens
+      integer(c_int32_t), value :: nsteps
+    end function
+
+    ! tangent-linear and adjoint (gradients with respect to the initial psi);
+    ! controls and gradients are physical real(c_double) X(ldx, n2, n3)
+    integer(c_int) function qg_adj_create(ctx, checkpoint_every, z_deep, adj) bind(C, name="qg_adj_create")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: checkpoint_every
+      real(c_double), value :: z_deep
+      type(c_ptr), intent(out) :: adj
+    end function
+
+    integer(c_int) function qg_adj_destroy(adj) bind(C, name="qg_adj_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: adj
+    end function
+
+    type(c_ptr) function qg_adj_error(adj) bind(C, name="qg_adj_error")
+      import :: c_ptr
+      type(c_ptr), value :: adj
+    end function
+
+    integer(c_int) function qg_adj_set_fused(adj, fused) bind(C, name="qg_adj_set_fused")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: adj
+      integer(c_int32_t), value :: fused
+    end function
+
+    integer(c_int) function qg_adj_objective(adj, psir, BRr, BIr, ldx, nsteps, J) bind(C, name="qg_adj_objective")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: adj, psir, BRr, BIr, J
+      integer(c_int32_t), value :: ldx, nsteps
+    end function
+
+    integer(c_int) function qg_adj_tlm(adj, psir, BRr, BIr, ldx, nsteps, dpsir, dJ, dstate) bind(C, name="qg_adj_tlm")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: adj, psir, BRr, BIr, dpsir, dJ, dstate
+      integer(c_int32_t), value :: ldx, nsteps
+    end function
+
+    integer(c_int) function qg_adj_apply(adj, psir, BRr, BIr, ldx, nsteps, state_bar, psi_bar) &
+        bind(C, name="qg_adj_apply")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: adj, psir, BRr, BIr, state_bar, psi_bar
+      integer(c_int32_t), value :: ldx, nsteps
+    end function
+
+    integer(c_int) function qg_adj_gradient(adj, psir, BRr, BIr, ldx, nsteps, J, grad) bind(C, name="qg_adj_gradient")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: adj, psir, BRr, BIr, J, grad
+      integer(c_int32_t), value :: ldx, nsteps
+    end function
   end interface
 
 contains
 
-  ! message of the last failure on a context, as a Fortran string
-  function qg_error_string(ctx) result(msg)
-    type(c_ptr), intent(in) :: ctx
+  ! a NUL-terminated C string (qg_last_error, qg_ensemble_error,
+  ! qg_adj_error) as a Fortran string
+  function c_string(p) result(msg)
+    type(c_ptr), intent(in) :: p
     character(len=256) :: msg
     character(kind=c_char), pointer :: cs(:)
-    type(c_ptr) :: p
     integer :: i
     msg = ' '
-    p = qg_last_error(ctx)
     if (.not. c_associated(p)) return
     call c_f_pointer(p, cs, [256])
     do i = 1, 256
       if (cs(i) == c_null_char) exit
       msg(i:i) = cs(i)
     end do
+  end function c_string
+
+  ! message of the last failure on a context, as a Fortran string
+  function qg_error_string(ctx) result(msg)
+    type(c_ptr), intent(in) :: ctx
+    character(len=256) :: msg
+    msg = c_string(qg_last_error(ctx))
   end function qg_error_string
 
 end module qgybj_c

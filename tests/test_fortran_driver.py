@@ -104,3 +104,31 @@ def test_driver_run_matches_oracle(tmp_path, n1, n3, flags):
     assert sorted(seen) == [0, 5, 10]
     for it, d in diags.items():
         np.testing.assert_allclose(seen[it], d, rtol=1e-11, atol=1e-14 * np.max(np.abs(d)))
+
+
+@pytest.mark.gpu
+def test_driver_gradient_matches_harness(tmp_path):
+    """The driver's gradient mode (Fortran -> qg_adj_* over ISO_C_BINDING)
+    returns the objective and gradient the Python harness gets from the same
+    library on the same initial conditions, bit for bit."""
+    import qgybj
+    n1, n3, nsteps, every = 32, 16, 8, 3
+    P = make_params(n1, n3, dt=2e-3, W2F=0.05)
+    psi, BR, BI = physical_ics(P, k0=3, seed=4)
+    for name, a in (("psi", psi), ("BR", BR), ("BI", BI)):
+        np.ascontiguousarray(a, dtype=np.float64).tofile(os.path.join(tmp_path, f"ic_{name}.in"))
+    print(f"fortran driver gradient n1={n1} n3={n3} steps={nsteps}", flush=True)
+    r = _run(tmp_path, n1=n1, n3=n3, delt=2e-3, W2F=0.05, ic_prefix=os.path.join(tmp_path, "ic_"),
+             out_prefix=os.path.join(tmp_path, "out_"), grad_steps=nsteps, grad_every=every)
+    assert r.returncode == 0, r.stdout + r.stderr
+    line = [ln for ln in r.stdout.splitlines() if ln.startswith("# grad steps")]
+    assert line, r.stdout
+    J = float(line[0].split()[-1])
+    g = np.fromfile(os.path.join(tmp_path, "out_grad.bin"), dtype=np.float64).reshape(n3, n1, n1)
+    m = qgybj.Member(P)
+    adj = qgybj.Adjoint(m, checkpoint_every=every)
+    J2, g2 = adj.gradient(psi, BR, BI, nsteps)
+    adj.close()
+    m.close()
+    assert J == J2, (J, J2)
+    assert np.array_equal(g, g2)
