@@ -138,7 +138,7 @@ def test_torch_autograd_harness():
     J = wave_energy(x, adj, BR, BI, nsteps)
     (2.0 * J).backward()
     J2, g = adj.gradient(psi, BR, BI, nsteps)
-    assert float(J) == J2
+    assert float(J.detach()) == J2
     np.testing.assert_array_equal(x.grad.numpy(), 2.0 * g)
     adj.close()
     m.close()
