@@ -333,6 +333,8 @@ int qg_create_batch(const qg_params* p, int32_t members, qg_ctx** out) {
   d.tune.a_main = 0;
   d.tune.e16[2] = 2;  // col_upd with one exchange buffer (171 vs 182 us)
   d.tune.e16[4] = 2;  // col_qw with one exchange buffer (43 vs 48 us)
+  // at 512^2 x 256, col_qw with 16 elements per thread: 327 vs 511 us
+  if (p->n2 >= 512) d.tune.e16[4] = 3;
   d.flags = p->flags;
   d.dt = p->dt;
   d.gamma = p->gamma;
