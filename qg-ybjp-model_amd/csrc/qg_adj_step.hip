@@ -81,8 +81,8 @@ __global__ __launch_bounds__((SH<N, E>::COLT)) void k_adj_col_qw(Dev d, AdjBufs 
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       const size_t o = pz + (size_t)(t + T * m) * g.sp + kx;
-      b.A[0][o] = v[0][m];
-      b.A[1][o] = v[1][m];
+      st_nt(&b.A[0][o], v[0][m]);
+      st_nt(&b.A[1][o], v[1][m]);
     }
   }
 }
@@ -160,8 +160,8 @@ __global__ __launch_bounds__(RP*(N / E)) void k_adj_row_qw(Dev d, AdjBufs b, dou
       const double kk = (double)k;
       kx_r[m] = mul_ik(Xrx, -kk);
       kx_i[m] = mul_ik(Xix, -kk);
-      b.A[3][ro + k] = Xry;
-      b.A[5][ro + k] = Xiy;
+      st_nt(&b.A[3][ro + k], Xry);
+      st_nt(&b.A[5][ro + k], Xiy);
     }
   }
   rf_publish_range<N, E, 3, 2, 1>(pv, xb, t);
@@ -171,8 +171,8 @@ __global__ __launch_bounds__(RP*(N / E)) void k_adj_row_qw(Dev d, AdjBufs b, dou
     if (k < KX) {
       double2 Xr, Xi;
       split2(pv[2][m], xb.pub(0, (N - k) & (N - 1)), Xr, Xi);
-      b.A[2][ro + k] = cadd(kx_r[m], Xr);
-      b.A[4][ro + k] = cadd(kx_i[m], Xi);
+      st_nt(&b.A[2][ro + k], cadd(kx_r[m], Xr));
+      st_nt(&b.A[4][ro + k], cadd(kx_i[m], Xi));
     }
   }
 }
@@ -272,8 +272,8 @@ __global__ __launch_bounds__((SH<N, E>::COLT)) void k_adj_col_upd(Dev d, AdjBufs
   if (colok) {
 #pragma unroll
     for (int m = 0; m < E; ++m) {
-      (Y0 + pz + m * mstride)[toff] = v[0][m];
-      (Y1 + pz + m * mstride)[toff] = v[1][m];
+      st_nt(&(Y0 + pz + m * mstride)[toff], v[0][m]);
+      st_nt(&(Y1 + pz + m * mstride)[toff], v[1][m]);
     }
   }
 }
@@ -359,10 +359,10 @@ __global__ __launch_bounds__(RP*(N / E)) void k_adj_row_nl(Dev d, AdjBufs b, int
       split2(pv[1][m], xb.pub(1, km), Xz, Xq);
       const double kk = (double)k;
       const double2 a1 = mul_ik(Xv, -kk);
-      b.A[0][ro + k] = Xu;
-      b.A[1][ro + k] = make_double2(a1.x - kk * kk * Xz.x, a1.y - kk * kk * Xz.y);
-      b.A[2][ro + k] = Xz;
-      b.A[3][ro + k] = Xq;
+      st_nt(&b.A[0][ro + k], Xu);
+      st_nt(&b.A[1][ro + k], make_double2(a1.x - kk * kk * Xz.x, a1.y - kk * kk * Xz.y));
+      st_nt(&b.A[2][ro + k], Xz);
+      st_nt(&b.A[3][ro + k], Xq);
     }
   }
   rf_publish_range<N, E, 3, 2, 1>(pv, xb, t);
@@ -372,8 +372,8 @@ __global__ __launch_bounds__(RP*(N / E)) void k_adj_row_nl(Dev d, AdjBufs b, int
     if (k < KX) {
       double2 Xr, Xi;
       split2(pv[2][m], xb.pub(0, (N - k) & (N - 1)), Xr, Xi);
-      b.A[4][ro + k] = Xr;
-      b.A[5][ro + k] = Xi;
+      st_nt(&b.A[4][ro + k], Xr);
+      st_nt(&b.A[5][ro + k], Xi);
     }
   }
 }

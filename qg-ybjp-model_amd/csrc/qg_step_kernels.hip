@@ -85,7 +85,7 @@ __global__ __launch_bounds__((SH<N, E>::COLT)) void k_ps_inv(Geo g, const double
     reg_fft<N, E, true>(v, xb, t, tw);
     if (kx < W) {
 #pragma unroll
-      for (int m = 0; m < E; ++m) out[pz + (size_t)(t + T * m) * g.sp + kx] = v[m];
+      for (int m = 0; m < E; ++m) st_nt(&out[pz + (size_t)(t + T * m) * g.sp + kx], v[m]);
     }
   }
 }
@@ -173,8 +173,8 @@ __global__ __launch_bounds__(RP*(N / E)) void k_row_nl(Dev d, int Win, double sc
       if (lin) { w2 = zero2(); w3 = zero2(); }
       if (!pas) w2 = make_double2(w2.x - 0.5 * FzI.x, w2.y - 0.5 * FzI.y);
       keepZR[m] = pas ? zero2() : FzR;
-      I[2][ro + k] = cscale(w2, scale);
-      I[3][ro + k] = cscale(w3, scale);
+      st_nt(&I[2][ro + k], cscale(w2, scale));
+      st_nt(&I[3][ro + k], cscale(w3, scale));
     }
   }
   // second pair: I4 = i k F(u BI) + F(zeta BR)/2, I5 = F(v BI),
@@ -192,8 +192,10 @@ __global__ __launch_bounds__(RP*(N / E)) void k_row_nl(Dev d, int Win, double sc
       double2 w0 = mul_ik(Fuq, kk), w1 = Fvq, w4 = mul_ik(FuI, kk), w5 = FvI;
       if (lin) { w0 = zero2(); w1 = zero2(); w4 = zero2(); w5 = zero2(); }
       w4 = make_double2(w4.x + 0.5 * keepZR[m].x, w4.y + 0.5 * keepZR[m].y);
-      I[0][ro + k] = cscale(w0, scale); I[1][ro + k] = cscale(w1, scale);
-      I[4][ro + k] = cscale(w4, scale); I[5][ro + k] = cscale(w5, scale);
+      st_nt(&I[0][ro + k], cscale(w0, scale));
+      st_nt(&I[1][ro + k], cscale(w1, scale));
+      st_nt(&I[4][ro + k], cscale(w4, scale));
+      st_nt(&I[5][ro + k], cscale(w5, scale));
     }
   }
 }
@@ -327,10 +329,10 @@ __global__ __launch_bounds__((SH<N, E>::COLT)) void k_col_upd(Dev d, int euler, 
   reg_fft_multi<N, E, 2, SB, true>(v, xb, t, d.tw);
   if (colok) {
 #pragma unroll
-    for (int m = 0; m < E; ++m) (W1 + pz + m * mstride)[toff] = v[0][m];
+    for (int m = 0; m < E; ++m) st_nt(&(W1 + pz + m * mstride)[toff], v[0][m]);
     if (f > 0) {
 #pragma unroll
-      for (int m = 0; m < E; ++m) (W2 + pz + m * mstride)[toff] = v[1][m];
+      for (int m = 0; m < E; ++m) st_nt(&(W2 + pz + m * mstride)[toff], v[1][m]);
     }
   }
 }
@@ -381,8 +383,8 @@ __global__ __launch_bounds__(RP*(N / E)) void k_row_qw(Dev d, double scale) {
       double2 FJ, FM;
       split2(pv[0][m], xb.pub(0, (N - k) & (N - 1)), FJ, FM);
       const double c4 = 0.25 * (double)k * (double)k;
-      I[3][ro + k] = cscale(make_double2(FJ.x - c4 * FM.x, FJ.y - c4 * FM.y), scale);
-      I[5][ro + k] = cscale(FM, scale);
+      st_nt(&I[3][ro + k], cscale(make_double2(FJ.x - c4 * FM.x, FJ.y - c4 * FM.y), scale));
+      st_nt(&I[5][ro + k], cscale(FM, scale));
     }
   }
 }

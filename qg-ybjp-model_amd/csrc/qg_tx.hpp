@@ -8,6 +8,18 @@
 namespace qg {
 
 __device__ __forceinline__ double2 zero2() { return make_double2(0.0, 0.0); }
+
+// Streaming store of an intermediate (kx, y) plane, which only the next
+// transform kernel reads: the non-temporal hint writes it through instead of
+// leaving dirty lines to be written back while the next kernel streams.  A
+// memory-only kernel with the row kernels' access pattern goes from 4.3 to
+// 5.9 TB/s with it; on the whole step (every plane store non-temporal, the
+// state fields and solve outputs not, which measured slower) 0.495 ->
+// 0.479 ms at 256^2 x 128.
+__device__ __forceinline__ void st_nt(double2* p, double2 v) {
+  __builtin_nontemporal_store(v.x, &p->x);
+  __builtin_nontemporal_store(v.y, &p->y);
+}
 __device__ __forceinline__ double2 mul_ik(double2 v, double k) { return make_double2(-k * v.y, k * v.x); }
 
 // Pack entry k of two real half-spectra A, B into position pos of a complex
