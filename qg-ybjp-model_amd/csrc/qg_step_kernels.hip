@@ -251,13 +251,13 @@ __global__ __launch_bounds__((SH<N, E>::COLT)) void k_col_upd(Dev d, int euler, 
       double2 xonew = zero2();
       if (lmask(kx, iky, g)) {
         const size_t kidx = (size_t)iky * g.iktx + kx;  // (iky, kx) tables are unpadded
-        const double2 x = X[idx];
+        const double2 x = ld_nt(&X[idx]);
         const double e1 = e1t[kidx];
         const double kh2 = (double)kx * (double)kx + ky * ky;
         double2 dqv = zero2();
         if (f == 0 && has_dq) dqv = d.dq[idx];
         double2 av = zero2();
-        if (useA) av = Av[idx];
+        if (useA) av = ld_nt(&Av[idx]);
         if (euler) {
           if (f == 0) {
             xnew = make_double2(((x.x - d.dt * tt.x) + d.dt * dqv.x) * e1, ((x.y - d.dt * tt.y) + d.dt * dqv.y) * e1);
@@ -267,7 +267,7 @@ __global__ __launch_bounds__((SH<N, E>::COLT)) void k_col_upd(Dev d, int euler, 
                                 ((x.y - d.dt * tt.y) - sgn * (ck * av.y)) * e1);
           }
         } else {
-          const double2 xo = Xo[idx];
+          const double2 xo = ld_nt(&Xo[idx]);
           const double e2 = e2t[kidx];
           const double tdt = 2.0 * d.dt;
           if (f == 0) {
