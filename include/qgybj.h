@@ -153,6 +153,11 @@ int qg_sync(qg_ctx* ctx);
  * throughput itself is being measured. */
 enum { QG_TIME_ALL = 0xFF };
 int qg_timing(qg_ctx* ctx, int32_t mask);
+/* Time only every `every`-th launch of each selected kernel (default 1: all),
+ * counting from the call; fewer event records on the stream while the
+ * average launch duration is still measured inside the timed run.  No
+ * counterpart in the reference. */
+int qg_timing_sample(qg_ctx* ctx, int32_t every);
 int qg_kernel_times(qg_ctx* ctx, double* ms, int32_t* counts, int32_t n);
 const char* qg_kernel_name(int32_t id);
 

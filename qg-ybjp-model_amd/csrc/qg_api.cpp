@@ -124,7 +124,9 @@ struct Timed {
   hipStream_t st;
   hipEvent_t a = nullptr;
   Timed(qg_ctx* cc, int i, hipStream_t s) : c(cc), id(i), st(s) {
-    if (!c->capturing && ((c->timing >> id) & 1u) && hipEventCreate(&a) == hipSuccess) hipEventRecord(a, st);
+    if (c->capturing || !((c->timing >> id) & 1u)) return;
+    if (c->tseen[id]++ % c->timing_every != 0) return;
+    if (hipEventCreate(&a) == hipSuccess) hipEventRecord(a, st);
   }
   void done() {
     if (a) {
@@ -638,6 +640,13 @@ int qg_sync(qg_ctx* c) {
 int qg_timing(qg_ctx* c, int32_t mask) {
   if (!c) return QG_ERR_ARG;
   c->timing = (uint32_t)mask & ((1u << qg::K_NKERNELS) - 1u);
+  return QG_OK;
+}
+
+int qg_timing_sample(qg_ctx* c, int32_t every) {
+  if (!c || every < 1) return QG_ERR_ARG;
+  c->timing_every = every;
+  for (int i = 0; i < qg::K_NKERNELS; ++i) c->tseen[i] = 0;
   return QG_OK;
 }
 

@@ -24,6 +24,8 @@ struct qg_ctx {
   std::string err;
   std::vector<void*> allocs;
   uint32_t timing = 0;  // bit k: time kernel id k
+  int timing_every = 1;  // time every timing_every-th launch of a kernel
+  int tseen[qg::K_NKERNELS] = {};  // launches of each kernel since qg_timing_sample
   struct Ev {
     int id;
     hipEvent_t a, b;

@@ -29,7 +29,7 @@ KERNEL_NAMES = ["col_inv", "row_nl", "col_upd", "row_qw", "col_qw", "solve", "so
 SYMBOLS = [
     "qg_create", "qg_create_batch", "qg_members", "qg_destroy", "qg_last_error", "qg_set_state", "qg_set_state_physical", "qg_get_state",
     "qg_step", "qg_iter", "qg_diag", "qg_init_q", "qg_fft_r2c", "qg_fft_c2r", "qg_device_field", "qg_device_pitch", "qg_stream",
-    "qg_sync", "qg_timing", "qg_kernel_times", "qg_kernel_name", "qg_set_variants", "qg_get_variants",
+    "qg_sync", "qg_timing", "qg_timing_sample", "qg_kernel_times", "qg_kernel_name", "qg_set_variants", "qg_get_variants",
     "qg_ensemble_create", "qg_ensemble_destroy", "qg_ensemble_error", "qg_ensemble_set_graph_steps",
     "qg_ensemble_step", "qg_tune", "qg_adj_create", "qg_adj_destroy", "qg_adj_error", "qg_adj_set_fused", "qg_adj_objective",
     "qg_adj_tlm", "qg_adj_apply", "qg_adj_gradient", "qg_mask", "qg_box", "qg_version",
@@ -85,6 +85,7 @@ def load(path: str = LIB_PATH):
     lib.qg_stream.argtypes = [vp, C.POINTER(vp)]
     lib.qg_sync.argtypes = [vp]
     lib.qg_timing.argtypes = [vp, C.c_int32]
+    lib.qg_timing_sample.argtypes = [vp, C.c_int32]
     lib.qg_kernel_times.argtypes = [vp, dp, C.POINTER(C.c_int32), C.c_int32]
     lib.qg_kernel_name.argtypes = [C.c_int32]
     lib.qg_kernel_name.restype = C.c_char_p
@@ -278,6 +279,10 @@ class Member:
         else:
             mask = int(on)
         self._chk(self.lib.qg_timing(self._h, mask), "qg_timing")
+
+    def timing_sample(self, every: int = 1):
+        """Time only every `every`-th launch of each timed kernel."""
+        self._chk(self.lib.qg_timing_sample(self._h, int(every)), "qg_timing_sample")
 
     def kernel_times(self):
         n = len(KERNEL_NAMES)

@@ -269,3 +269,36 @@ def test_set_variants_rejects_out_of_range():
         m.set_variants([0, 0, 0, 0, 0, 0, 2])
     assert list(m.get_variants()) == before
     m.close()
+
+
+def test_kernel_timing_and_sampling():
+    """qg_timing times the selected kernels with events on their streams;
+    qg_timing_sample(k) times every k-th launch; timing never changes the
+    results."""
+    P = make_params(64, 8, W2F=0.05, dt=2e-3)
+    psi, BR, BI = physical_ics(P, k0=3)
+    ref = _member(P)
+    ref.set_state_physical(psi, BR, BI)
+    ref.step(11)
+    m = _member(P)
+    m.set_state_physical(psi, BR, BI)
+    m.step(2)
+    m.kernel_times()
+    m.timing(True)
+    m.step(3)
+    kt = m.kernel_times()
+    assert kt["row_nl"][1] == 3 and kt["col_upd"][1] == 3 and kt["row_nl"][0] > 0
+    m.timing_sample(3)
+    m.timing("col_upd")
+    m.step(6)
+    m.timing(False)
+    m.timing_sample(1)
+    kt = m.kernel_times()
+    assert kt["col_upd"][1] == 2 and kt["row_nl"][1] == 0
+    a, b = m.get_state(), ref.get_state()
+    for f in qgybj.Member.FIELDS:
+        assert np.array_equal(a[f], b[f]), f
+    with pytest.raises(qgybj.QGError):
+        m.timing_sample(0)
+    m.close()
+    ref.close()
