@@ -329,9 +329,10 @@ int qg_create_batch(const qg_params* p, int32_t members, qg_ctx** out) {
   Dev& d = c->d;
   d.g = qg::make_geo(p->n1, p->n2, p->n3, members);
   const Geo& g = d.g;
-  // measured defaults at 256^2 x 128: 16-level chunks, 32 columns per block,
-  // A inversions on the side stream
-  d.tune.solve = 3;
+  // measured defaults at 256^2 x 128: 8-level chunks, 32 columns per block
+  // (0.471 vs 0.474 ms per step with 16-level chunks), A inversions on the
+  // side stream
+  d.tune.solve = 1;
   d.tune.a_main = 0;
   d.tune.e16[2] = 2;  // col_upd with one exchange buffer (171 vs 182 us)
   d.tune.e16[4] = 2;  // col_qw with one exchange buffer (43 vs 48 us)
