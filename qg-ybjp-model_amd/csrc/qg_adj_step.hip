@@ -46,14 +46,21 @@ __device__ __forceinline__ void realit_col0(double2 (&v)[S][E], const ColBuf<N, 
       }
   }
 }
+// the same for all S sequences through SB exchange buffers, SB at a time
+template <int N, int E, int S, int SB, int S0 = 0>
+__device__ __forceinline__ void realit_cols(double2 (&v)[S][E], const ColBuf<N, E>& xb, int t, int c) {
+  constexpr int G = (S - S0) < SB ? (S - S0) : SB;
+  realit_col0<N, E, S, S0, G>(v, xb, t, c);
+  if constexpr (S0 + G < S) realit_cols<N, E, S, SB, S0 + G>(v, xb, t, c);
+}
 
 // ---------------------------------------------------------------------------
 // adj_col_qw: y-inverses of the feedback adjoint seeds
 // ---------------------------------------------------------------------------
-template <int N, int E>
+template <int N, int E, int SB>
 __global__ __launch_bounds__((SH<N, E>::COLT)) void k_adj_col_qw(Dev d, AdjBufs b) {
   constexpr int T = SH<N, E>::T, TX = SH<N, E>::TX;
-  __shared__ double2 lds[col_xlen<N, E, 2, TX>()];
+  __shared__ double2 lds[col_xlen<N, E, SB, TX>()];
   const Geo& g = d.g;
   const int c = threadIdx.x % TX, t = threadIdx.x / TX;
   const int kx0 = blockIdx.x * TX, kx = kx0 + c, z = blockIdx.y;
@@ -75,8 +82,8 @@ __global__ __launch_bounds__((SH<N, E>::COLT)) void k_adj_col_qw(Dev d, AdjBufs 
     v[0][m] = a;
     v[1][m] = cscale(a, (double)ky * (double)ky);
   }
-  if (kx0 == 0) realit_col0<N, E, 2, 0, 2>(v, xb, t, c);
-  reg_fft_multi<N, E, 2, 2, true>(v, xb, t, d.tw);
+  if (kx0 == 0) realit_cols<N, E, 2, SB>(v, xb, t, c);
+  reg_fft_multi<N, E, 2, SB, true>(v, xb, t, d.tw);
   if (colok) {
 #pragma unroll
     for (int m = 0; m < E; ++m) {
@@ -184,10 +191,10 @@ __global__ __launch_bounds__(RP*(N / E)) void k_adj_row_qw(Dev d, AdjBufs b, dou
 // the state adjoint in place, and form the y-inverses of the tendency seeds
 // T/W and (-i ky) T/W.  blockIdx.z: 0 = q, 1 = BR, 2 = BI.
 // ---------------------------------------------------------------------------
-template <int N, int E>
+template <int N, int E, int SB>
 __global__ __launch_bounds__((SH<N, E>::COLT)) void k_adj_col_upd(Dev d, AdjBufs b, int euler, int feedback) {
   constexpr int T = SH<N, E>::T, TX = SH<N, E>::TX;
-  __shared__ double2 lds[col_xlen<N, E, 2, TX>()];
+  __shared__ double2 lds[col_xlen<N, E, SB, TX>()];
   const Geo& g = d.g;
   const int c = threadIdx.x % TX, t = threadIdx.x / TX;
   const int kx0 = blockIdx.x * TX, kx = kx0 + c, z = blockIdx.y, f = blockIdx.z;
@@ -206,7 +213,7 @@ __global__ __launch_bounds__((SH<N, E>::COLT)) void k_adj_col_upd(Dev d, AdjBufs
       v[0][m] = colok ? (Y0 + pz + m * mstride)[toff] : zero2();
       v[1][m] = colok ? (Y1 + pz + m * mstride)[toff] : zero2();
     }
-    reg_fft_multi<N, E, 2, 2, false>(v, xb, t, d.tw);
+    reg_fft_multi<N, E, 2, SB, false>(v, xb, t, d.tw);
   }
   double2* X = f == 0 ? b.q : (f == 1 ? b.BR : b.BI);
   double2* Xo = f == 0 ? b.qo : (f == 1 ? b.BRo : b.BIo);
@@ -267,8 +274,8 @@ __global__ __launch_bounds__((SH<N, E>::COLT)) void k_adj_col_upd(Dev d, AdjBufs
     v[0][m] = o0;
     v[1][m] = o1;
   }
-  if (kx0 == 0) realit_col0<N, E, 2, 0, 2>(v, xb, t, c);
-  reg_fft_multi<N, E, 2, 2, true>(v, xb, t, d.tw);
+  if (kx0 == 0) realit_cols<N, E, 2, SB>(v, xb, t, c);
+  reg_fft_multi<N, E, 2, SB, true>(v, xb, t, d.tw);
   if (colok) {
 #pragma unroll
     for (int m = 0; m < E; ++m) {
@@ -384,10 +391,10 @@ __global__ __launch_bounds__(RP*(N / E)) void k_adj_row_nl(Dev d, AdjBufs b, int
 // qbar += W L Y(A3), BRbar += W L Y(A4), BIbar += W L Y(A5).
 // blockIdx.z: 0 = psi, 1 = q, BR, BI.
 // ---------------------------------------------------------------------------
-template <int N, int E>
+template <int N, int E, int SB>
 __global__ __launch_bounds__((SH<N, E>::COLT)) void k_adj_col_out(Dev d, AdjBufs b) {
   constexpr int T = SH<N, E>::T, TX = SH<N, E>::TX;
-  __shared__ double2 lds[col_xlen<N, E, 2, TX>()];
+  __shared__ double2 lds[col_xlen<N, E, SB, TX>()];
   const Geo& g = d.g;
   const int c = threadIdx.x % TX, t = threadIdx.x / TX;
   const int kx0 = blockIdx.x * TX, kx = kx0 + c, z = blockIdx.y, f = blockIdx.z;
@@ -401,7 +408,7 @@ __global__ __launch_bounds__((SH<N, E>::COLT)) void k_adj_col_out(Dev d, AdjBufs
   for (int s = 0; s < 3; ++s)
 #pragma unroll
     for (int m = 0; m < E; ++m) v[s][m] = colok ? (b.A[3 * f + s] + pz + m * mstride)[toff] : zero2();
-  reg_fft_multi<N, E, 3, 2, false>(v, xb, t, d.tw);
+  reg_fft_multi<N, E, 3, SB, false>(v, xb, t, d.tw);
   if (!colok) return;
 #pragma unroll
   for (int m = 0; m < E; ++m) {
@@ -429,8 +436,12 @@ struct LA {
   static constexpr int E = 8;
   using S = SH<N, E>;
   static constexpr int RP = S::template rp<64>();
+  static constexpr int CSB = 1;  // exchange buffers of the column kernels (LDS for 3 waves)
+  // the row_nl kernel holds seven transforms; at E = 8 they take every
+  // register (1 wave per SIMD), at E = 4 half as many (2 waves)
+  static constexpr int ENL = N >= 128 ? 4 : 8, RPNL = SH<N, ENL>::template rp<64>();
   static hipError_t col_qw(const Dev& d, hipStream_t st, const AdjBufs& b) {
-    hipLaunchKernelGGL((k_adj_col_qw<N, E>), dim3((d.g.KX + S::TX - 1) / S::TX, d.g.nz), dim3(S::COLT), 0, st, d, b);
+    hipLaunchKernelGGL((k_adj_col_qw<N, E, CSB>), dim3((d.g.KX + S::TX - 1) / S::TX, d.g.nz), dim3(S::COLT), 0, st, d, b);
     return hipGetLastError();
   }
   static hipError_t row_qw(const Dev& d, hipStream_t st, const AdjBufs& b) {
@@ -441,19 +452,19 @@ struct LA {
     return hipGetLastError();
   }
   static hipError_t col_upd(const Dev& d, hipStream_t st, const AdjBufs& b, int euler, int feedback) {
-    hipLaunchKernelGGL((k_adj_col_upd<N, E>), dim3((d.g.KX + S::TX - 1) / S::TX, d.g.nz, 3), dim3(S::COLT), 0, st, d,
+    hipLaunchKernelGGL((k_adj_col_upd<N, E, CSB>), dim3((d.g.KX + S::TX - 1) / S::TX, d.g.nz, 3), dim3(S::COLT), 0, st, d,
                        b, euler, feedback);
     return hipGetLastError();
   }
   static hipError_t row_nl(const Dev& d, hipStream_t st, const AdjBufs& b, int Win, int euler) {
     const double Nn = (double)d.g.n1 * (double)d.g.n2;
     const double sg = euler ? d.dt : 2.0 * d.dt;
-    hipLaunchKernelGGL((k_adj_row_nl<N, E, RP, 2>), dim3(d.g.n2 * d.g.nz / RP), dim3(RP * (N / E)), 0, st, d, b, Win,
-                       -sg / Nn, sg / (2.0 * Nn));
+    hipLaunchKernelGGL((k_adj_row_nl<N, ENL, RPNL, 2>), dim3(d.g.n2 * d.g.nz / RPNL), dim3(RPNL * (N / ENL)), 0, st, d,
+                       b, Win, -sg / Nn, sg / (2.0 * Nn));
     return hipGetLastError();
   }
   static hipError_t col_out(const Dev& d, hipStream_t st, const AdjBufs& b) {
-    hipLaunchKernelGGL((k_adj_col_out<N, E>), dim3((d.g.KX + S::TX - 1) / S::TX, d.g.nz, 2), dim3(S::COLT), 0, st, d,
+    hipLaunchKernelGGL((k_adj_col_out<N, E, CSB>), dim3((d.g.KX + S::TX - 1) / S::TX, d.g.nz, 2), dim3(S::COLT), 0, st, d,
                        b);
     return hipGetLastError();
   }

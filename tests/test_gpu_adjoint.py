@@ -183,10 +183,12 @@ def psi_shape(P):
     return (P.n3, P.n1, P.n1)
 
 
-@pytest.mark.parametrize("n1,n3,flags", [(32, 16, 0), (64, 8, qgybj.params.NO_FEEDBACK), (16, 40, 0)])
+@pytest.mark.parametrize("n1,n3,flags", [(32, 16, 0), (64, 8, qgybj.params.NO_FEEDBACK), (16, 40, 0), (128, 8, 0),
+                                          (256, 4, qgybj.params.NO_FEEDBACK)])
 def test_fused_adjoint_matches_generic(n1, n3, flags):
     """The fused adjoint kernels and the generic (pointwise + separate
-    transforms) adjoint compute the same map."""
+    transforms) adjoint compute the same map (n1 >= 128 runs the row kernel
+    with 4 elements per thread)."""
     P, psi, BR, BI = _setup(n1=n1, n3=n3, flags=flags, seed=9)
     nsteps = 7
     m = qgybj.Member(P)
