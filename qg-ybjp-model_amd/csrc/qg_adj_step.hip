@@ -63,6 +63,7 @@ __global__ __launch_bounds__((SH<N, E>::COLT)) void k_adj_col_qw(Dev d, AdjBufs 
   __shared__ double2 lds[col_xlen<N, E, SB, TX>()];
   const Geo& g = d.g;
   const int c = threadIdx.x % TX, t = threadIdx.x / TX;
+  const TwSrc<N, E, (E <= 8)> tws(d.tw, t);  // before the data loads
   const int kx0 = blockIdx.x * TX, kx = kx0 + c, z = blockIdx.y;
   const ColBuf<N, E> xb{lds + c};
   const bool colok = kx < g.KX;
@@ -83,7 +84,7 @@ __global__ __launch_bounds__((SH<N, E>::COLT)) void k_adj_col_qw(Dev d, AdjBufs 
     v[1][m] = cscale(a, (double)ky * (double)ky);
   }
   if (kx0 == 0) realit_cols<N, E, 2, SB>(v, xb, t, c);
-  reg_fft_multi<N, E, 2, SB, true>(v, xb, t, d.tw);
+  reg_fft_multi<N, E, 2, SB, true>(v, xb, t, tws.get());
   if (colok) {
 #pragma unroll
     for (int m = 0; m < E; ++m) {
@@ -111,6 +112,7 @@ __global__ __launch_bounds__(RP*(N / E)) void k_adj_row_qw(Dev d, AdjBufs b, dou
   const Geo& g = d.g;
   int t, p;
   row_lane<T, RP>(threadIdx.x, t, p);
+  const TwSrc<N, E, (E <= 8)> tws(d.tw, t);  // before the data loads
   const RowBuf<N, E> xb{lds + p * RS};
   const size_t ro = ((size_t)blockIdx.x * RP + p) * g.sp;
   const int KX = g.KX;
@@ -140,7 +142,7 @@ __global__ __launch_bounds__(RP*(N / E)) void k_adj_row_qw(Dev d, AdjBufs b, dou
     const double2 je = make_double2(kk * kk * aj.x + ajy.x, kk * kk * aj.y + ajy.y);
     z[3][m] = pack2<N>(cscale(aj, cJ), cscale(je, cE), pos);
   }
-  reg_fft_multi<N, E, 4, SB, true>(z, xb, t, d.tw);
+  reg_fft_multi<N, E, 4, SB, true>(z, xb, t, tws.get());
   // z0 = (BR, BI), z1 = (BRx, BIx), z2 = (BRy, BIy), z3 = (Jbar, Ebar)
   double2 pv[3][E];
 #pragma unroll
@@ -151,7 +153,7 @@ __global__ __launch_bounds__(RP*(N / E)) void k_adj_row_qw(Dev d, AdjBufs b, dou
     pv[1][m] = make_double2(jb * bix, -jb * brx);           // (BRy bar, BIy bar)
     pv[2][m] = make_double2(2.0 * eb * br, 2.0 * eb * bi);  // (BR bar, BI bar)
   }
-  reg_fft_multi<N, E, 3, SB, false>(pv, xb, t, d.tw);
+  reg_fft_multi<N, E, 3, SB, false>(pv, xb, t, tws.get());
   double2 kx_r[MK], kx_i[MK];
   rf_publish_range<N, E, 3, 0, 2>(pv, xb, t);
 #pragma unroll
@@ -197,6 +199,7 @@ __global__ __launch_bounds__((SH<N, E>::COLT)) void k_adj_col_upd(Dev d, AdjBufs
   __shared__ double2 lds[col_xlen<N, E, SB, TX>()];
   const Geo& g = d.g;
   const int c = threadIdx.x % TX, t = threadIdx.x / TX;
+  const TwSrc<N, E, (E <= 8)> tws(d.tw, t);  // before the data loads
   const int kx0 = blockIdx.x * TX, kx = kx0 + c, z = blockIdx.y, f = blockIdx.z;
   const ColBuf<N, E> xb{lds + c};
   const bool colok = kx < g.KX;
@@ -213,7 +216,7 @@ __global__ __launch_bounds__((SH<N, E>::COLT)) void k_adj_col_upd(Dev d, AdjBufs
       v[0][m] = colok ? (Y0 + pz + m * mstride)[toff] : zero2();
       v[1][m] = colok ? (Y1 + pz + m * mstride)[toff] : zero2();
     }
-    reg_fft_multi<N, E, 2, SB, false>(v, xb, t, d.tw);
+    reg_fft_multi<N, E, 2, SB, false>(v, xb, t, tws.get());
   }
   double2* X = f == 0 ? b.q : (f == 1 ? b.BR : b.BI);
   double2* Xo = f == 0 ? b.qo : (f == 1 ? b.BRo : b.BIo);
@@ -275,7 +278,7 @@ __global__ __launch_bounds__((SH<N, E>::COLT)) void k_adj_col_upd(Dev d, AdjBufs
     v[1][m] = o1;
   }
   if (kx0 == 0) realit_cols<N, E, 2, SB>(v, xb, t, c);
-  reg_fft_multi<N, E, 2, SB, true>(v, xb, t, d.tw);
+  reg_fft_multi<N, E, 2, SB, true>(v, xb, t, tws.get());
   if (colok) {
 #pragma unroll
     for (int m = 0; m < E; ++m) {
@@ -304,6 +307,7 @@ __global__ __launch_bounds__(RP*(N / E)) void k_adj_row_nl(Dev d, AdjBufs b, int
   const Geo& g = d.g;
   int t, p;
   row_lane<T, RP>(threadIdx.x, t, p);
+  const TwSrc<N, E, (E <= 8)> tws(d.tw, t);  // before the data loads
   const RowBuf<N, E> xb{lds + p * RS};
   const size_t ro = ((size_t)blockIdx.x * RP + p) * g.sp;
   const int KX = g.KX;
@@ -337,7 +341,7 @@ __global__ __launch_bounds__(RP*(N / E)) void k_adj_row_nl(Dev d, AdjBufs b, int
     z[5][m] = pack2<N>(mul_ik(ai, -s1 * kk), cscale(aiy, s1), pos);
     z[6][m] = pack2<N>(cscale(ai, -s2), cscale(ar, s2), pos);
   }
-  reg_fft_multi<N, E, 7, SB, true>(z, xb, t, d.tw);
+  reg_fft_multi<N, E, 7, SB, true>(z, xb, t, tws.get());
   double2 pv[3][E];
 #pragma unroll
   for (int m = 0; m < E; ++m) {
@@ -354,7 +358,7 @@ __global__ __launch_bounds__(RP*(N / E)) void k_adj_row_nl(Dev d, AdjBufs b, int
     pv[1][m] = make_double2(zb, qb);
     pv[2][m] = make_double2(rb, ib);
   }
-  reg_fft_multi<N, E, 3, SB, false>(pv, xb, t, d.tw);
+  reg_fft_multi<N, E, 3, SB, false>(pv, xb, t, tws.get());
   rf_publish_range<N, E, 3, 0, 2>(pv, xb, t);
 #pragma unroll
   for (int m = 0; m < MK; ++m) {
@@ -397,6 +401,7 @@ __global__ __launch_bounds__((SH<N, E>::COLT)) void k_adj_col_out(Dev d, AdjBufs
   __shared__ double2 lds[col_xlen<N, E, SB, TX>()];
   const Geo& g = d.g;
   const int c = threadIdx.x % TX, t = threadIdx.x / TX;
+  const TwSrc<N, E, (E <= 8)> tws(d.tw, t);  // before the data loads
   const int kx0 = blockIdx.x * TX, kx = kx0 + c, z = blockIdx.y, f = blockIdx.z;
   const ColBuf<N, E> xb{lds + c};
   const bool colok = kx < g.KX;
@@ -408,7 +413,7 @@ __global__ __launch_bounds__((SH<N, E>::COLT)) void k_adj_col_out(Dev d, AdjBufs
   for (int s = 0; s < 3; ++s)
 #pragma unroll
     for (int m = 0; m < E; ++m) v[s][m] = colok ? (b.A[3 * f + s] + pz + m * mstride)[toff] : zero2();
-  reg_fft_multi<N, E, 3, SB, false>(v, xb, t, d.tw);
+  reg_fft_multi<N, E, 3, SB, false>(v, xb, t, tws.get());
   if (!colok) return;
 #pragma unroll
   for (int m = 0; m < E; ++m) {

@@ -179,6 +179,26 @@ struct TwB {
   }
 };
 
+// Twiddle source of a transform kernel.  PRE: the bases are loaded once at
+// kernel entry, before the data loads, and shared by every transform of the
+// kernel (otherwise each transform loads them after its data has arrived, one
+// more cache round trip per transform); costs the bases' registers for the
+// kernel's lifetime.  !PRE: each transform loads its own.
+template <int N, int E, bool PRE>
+struct TwSrc;
+template <int N, int E>
+struct TwSrc<N, E, true> {
+  TwB<N, E> tb;
+  __device__ __forceinline__ TwSrc(const double2* __restrict__ tw, int t) { tb.load(tw, t); }
+  __device__ __forceinline__ const TwB<N, E>& get() const { return tb; }
+};
+template <int N, int E>
+struct TwSrc<N, E, false> {
+  const double2* tw;
+  __device__ __forceinline__ TwSrc(const double2* __restrict__ tw_, int) : tw(tw_) {}
+  __device__ __forceinline__ const double2* get() const { return tw; }
+};
+
 // Passes PI .. P-2 (the middle passes) of the group of G sequences S0 ..
 // S0+G-1: read from LDS, twiddle, DFT, write back in Stockham order.
 template <int N, int E, int S, int G, int S0, bool INV, int ST, int PI>

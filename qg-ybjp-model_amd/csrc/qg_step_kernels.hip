@@ -52,6 +52,7 @@ __global__ __launch_bounds__((SH<N, E>::COLT)) void k_ps_inv(Geo g, const double
   const int kx0 = blockIdx.x * TX, kx = kx0 + c;
   if (kx0 >= W) return;  // the grid is sized for the widest job
   const ColBuf<N, E> xb{lds + c};
+  const TwSrc<N, E, (E <= 8)> tws(tw, t);  // before the data loads
   const double2* Fz = J.F + (size_t)z * g.ikty * g.sp;
   double2 v0[E];
 #pragma unroll
@@ -82,7 +83,7 @@ __global__ __launch_bounds__((SH<N, E>::COLT)) void k_ps_inv(Geo g, const double
         }
       }
     }
-    reg_fft<N, E, true>(v, xb, t, tw);
+    reg_fft<N, E, true>(v, xb, t, tws.get());
     if (kx < W) {
 #pragma unroll
       for (int m = 0; m < E; ++m) st_nt(&out[pz + (size_t)(t + T * m) * g.sp + kx], v[m]);
@@ -112,6 +113,7 @@ __global__ __launch_bounds__(RP*(N / E)) void k_row_nl(Dev d, int Win, double sc
   const RowBuf<N, E> xb{lds + p * RS};
   const size_t ro = ((size_t)blockIdx.x * RP + p) * g.sp;
   double2* const* I = d.I;
+  const TwSrc<N, E, (E <= 8)> tws(d.tw, t);  // before the data loads
   // All loads are issued before any is used: positions with k >= Win read
   // entry 0 of the row (a valid address, same line as the others) and are
   // zeroed afterwards, so no branch splits the loads (a zero-initialised
@@ -141,7 +143,7 @@ __global__ __launch_bounds__(RP*(N / E)) void k_row_nl(Dev d, int Win, double sc
     z[1][m] = pack2<N>(Z, pq, pos);
     z[2][m] = pack2<N>(pbr, pbi, pos);
   }
-  reg_fft_multi<N, E, 3, SB, true>(z, xb, t, d.tw);
+  reg_fft_multi<N, E, 3, SB, true>(z, xb, t, tws.get());
   // z0 = (u, v), z1 = (zeta, q), z2 = (BR, BI) at positions t + T m; the
   // product pairs, ordered so that each exchanged pair completes outputs:
   // (zeta BR, zeta BI), (u BR, v BR) | (u BI, v BI), (u q, v q)
@@ -154,7 +156,7 @@ __global__ __launch_bounds__(RP*(N / E)) void k_row_nl(Dev d, int Win, double sc
     pv[2][m] = make_double2(u * bi, v * bi);
     pv[3][m] = make_double2(u * q, v * q);
   }
-  reg_fft_multi<N, E, 4, SB, false>(pv, xb, t, d.tw);
+  reg_fft_multi<N, E, 4, SB, false>(pv, xb, t, tws.get());
   const bool lin = (d.flags & F_LINEAR) != 0, pas = (d.flags & F_PASSIVE) != 0;
   const int KX = g.KX;
   double2 keepZR[MK];  // F(zeta BR)
@@ -355,6 +357,7 @@ __global__ __launch_bounds__(RP*(N / E)) void k_row_qw(Dev d, double scale) {
   const size_t ro = ((size_t)blockIdx.x * RP + p) * g.sp;
   double2* const* I = d.I;
   const int KX = g.KX;
+  const TwSrc<N, E, (E <= 8)> tws(d.tw, t);  // before the data loads
   double2 z[3][E];
 #pragma unroll
   for (int m = 0; m < E; ++m) {
@@ -367,14 +370,14 @@ __global__ __launch_bounds__(RP*(N / E)) void k_row_qw(Dev d, double scale) {
     z[1][m] = pack2<N>(mul_ik(br, kk), mul_ik(bi, kk), pos);
     z[2][m] = pack2<N>(bry, biy, pos);
   }
-  reg_fft_multi<N, E, 3, SB, true>(z, xb, t, d.tw);
+  reg_fft_multi<N, E, 3, SB, true>(z, xb, t, tws.get());
   // z0 = (BR, BI), z1 = (BRx, BIx), z2 = (BRy, BIy): (J, |B|^2)
   double2 pv[1][E];
 #pragma unroll
   for (int m = 0; m < E; ++m)
     pv[0][m] = make_double2(z[2][m].x * z[1][m].y - z[1][m].x * z[2][m].y,
                             z[0][m].x * z[0][m].x + z[0][m].y * z[0][m].y);
-  reg_fft_multi<N, E, 1, 1, false>(pv, xb, t, d.tw);
+  reg_fft_multi<N, E, 1, 1, false>(pv, xb, t, tws.get());
   rf_publish_range<N, E, 1, 0, 1>(pv, xb, t);
 #pragma unroll
   for (int m = 0; m < MK; ++m) {
