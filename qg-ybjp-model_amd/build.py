@@ -5,6 +5,7 @@
 Outputs:
     qg-ybjp-model_amd/lib/libqgybj.so     the C-ABI library (include/qgybj.h)
     qg-ybjp-model_amd/lib/qg_driver       the Fortran integration-loop driver
+    qg-ybjp-model_amd/lib/libqgncio.so    its classic-netCDF reader/writer (host tests)
 """
 from __future__ import annotations
 
@@ -26,6 +27,7 @@ INC = os.path.join(ROOT, "include")
 ARCH = "gfx950"
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 FC = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
+CC = shutil.which("gcc") or shutil.which("cc") or "/opt/rocm/llvm/bin/clang"
 
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
@@ -66,15 +68,29 @@ def build_library(jobs: int = 4, verbose: bool = False) -> str:
     return so
 
 
+def build_ncio(verbose: bool = False) -> str:
+    """The driver's classic-netCDF reader/writer (fortran/qg_ncio.c): an
+    object for the driver and lib/libqgncio.so for the host tests."""
+    os.makedirs(OBJ, exist_ok=True)
+    src = os.path.join(FSRC, "qg_ncio.c")
+    obj = os.path.join(OBJ, "qg_ncio.o")
+    _run([CC, "-O2", "-std=c11", "-Wall", "-fPIC", "-c", src, "-o", obj], verbose)
+    so = os.path.join(LIB, "libqgncio.so")
+    _run([CC, "-shared", "-fPIC", obj, "-o", so + ".tmp"], verbose)
+    os.replace(so + ".tmp", so)
+    return obj
+
+
 def build_driver(verbose: bool = False) -> str | None:
     """Compile the Fortran driver against libqgybj.so (amdflang)."""
-    srcs = ["qg_params.f90", "qgybj_c.f90", "qg_driver.f90"]
+    srcs = ["qg_params.f90", "qgybj_c.f90", "qg_ncio.f90", "qg_driver.f90"]
     if not all(os.path.exists(os.path.join(FSRC, s)) for s in srcs):
         return None
+    ncio = build_ncio(verbose)
     exe = os.path.join(LIB, "qg_driver")
     mod = os.path.join(OBJ, "fmod")
     os.makedirs(mod, exist_ok=True)
-    cmd = [FC, "-O2", "-J", mod, *[os.path.join(FSRC, s) for s in srcs],
+    cmd = [FC, "-O2", "-J", mod, *[os.path.join(FSRC, s) for s in srcs], ncio,
            "-L", LIB, "-lqgybj", f"-Wl,-rpath,{LIB}", "-Wl,-rpath,$ORIGIN", "-o", exe]
     _run(cmd, verbose)
     return exe

@@ -9,17 +9,19 @@
 ! this program only forms the parameters (qg_run_params), moves the initial
 ! and output fields and reports.
 !
-! Differences from the reference, by design: parameters come from the
-! namelist /qg/ instead of a compile-time module; initial conditions are raw
-! float64 files or the analytic fields of generate_fields_stag
-! (init.f90:795-882) instead of netCDF; r-space dumps are raw float32 files
-! (<out_prefix>psiNNNN.bin, laNNNN.bin with BR then BI) in the layout of the
-! reference's netCDF variables (IO_ncf.f90:24-120), since no netCDF library is
-! available.
+! Input and output follow the reference's netCDF schema (IO_ncf.f90, through
+! qg_ncio): initial conditions psi000.in.nc / la000.in.nc when init_ncf_psi /
+! init_ncf_la = 1, dumps N2.out.nc, psiNNN.out.nc and laNNN.out.nc.  The files
+! are classic netCDF (CDF-2) rather than the reference's netCDF-4, since no
+! netCDF library is available; dimensions, variables and values are the same.
+! Differences by design: parameters come from the namelist /qg/ instead of a
+! compile-time module; without netCDF inputs the initial conditions are raw
+! float64 files (ic_prefix) or analytic fields.
 program qg_driver
   use, intrinsic :: iso_c_binding
   use qg_run_params
   use qgybj_c
+  use qg_ncio
   implicit none
 
   integer :: iktx, ikty, ix, iy, iz, it, nxt, u, ios, dump
@@ -75,6 +77,7 @@ program qg_driver
   p%N2_u = c_loc(N2u)
   p%N2_s = c_loc(N2s)
   write(*, '(a,5es25.17)') '# nuh1 nuh2 nuh1w nuh2w nuz ', nuh1, nuh2, nuh1w, nuh2w, p%nuz
+  write(*, '(a,4es25.17)') '# U_scale Uw_scale L_scale N0 ', U_scale, Uw_scale, L_scale, N0
   flush(6)
   rc = qg_create(p, ctx)
   if (rc /= QG_OK) then
@@ -102,6 +105,9 @@ program qg_driver
     call read_field(trim(ic_prefix)//'BR.in', BRr)
     call read_field(trim(ic_prefix)//'BI.in', BIr)
   end if
+  ! the netCDF initial conditions replace these fields (main_waqg.f90:137-139)
+  if (init_ncf_psi == 1) call read_ncf_psi()
+  if (init_ncf_la == 1) call read_ncf_la()
   if (grad_steps > 0) then
     call gradient_mode()
     rc = qg_destroy(ctx)
@@ -112,7 +118,8 @@ program qg_driver
 
   dump = 0
   if (diag_every > 0) call report(0)
-  if (out_every > 0) call dump_fields()
+  if (out_every > 0) call dump_n2()
+  if (out_every > 0) call dump_fields(0)
 
   ! Euler first step, then leapfrog (main_waqg.f90:197-467): advance in
   ! chunks that end at the next diagnostic or output step
@@ -127,7 +134,7 @@ program qg_driver
       if (mod(it, diag_every) == 0) call report(it)
     end if
     if (out_every > 0) then
-      if (mod(it, out_every) == 0) call dump_fields()
+      if (mod(it, out_every) == 0) call dump_fields(it)
     end if
   end do
 
@@ -209,30 +216,82 @@ contains
     write(*, '(i8,es16.8,6es25.16)') i, i*delt, d
   end subroutine report
 
-  ! psi and L+A in r-space, float32, as the reference's netCDF dumps
-  subroutine dump_fields()
+  ! psi (ncread_psi, IO_ncf.f90:223-293): dimensional FLOAT psi(x, y, z)
+  ! over the grid, divided by U_scale L_scale
+  subroutine read_ncf_psi()
+    real(c_float), allocatable, target :: c(:, :, :)
+    real(c_float) :: t
+    allocate(c(n1, n1, n3))
+    call nc_check(nc_read_field(init_ncf_psi_filename, 'psi', c), init_ncf_psi_filename)
+    call nc_check(nc_read_time(init_ncf_psi_filename, t), init_ncf_psi_filename)
+    write(*, '(a,es16.8,a)') '# psi read, file time ', t, ' s'
+    psir = c/(U_scale*L_scale)
+    deallocate(c)
+  end subroutine read_ncf_psi
+
+  ! L+A (ncread_la, IO_ncf.f90:298-371): LAr, LAi divided by Uw_scale
+  subroutine read_ncf_la()
+    real(c_float), allocatable, target :: c(:, :, :)
+    real(c_float) :: t
+    allocate(c(n1, n1, n3))
+    call nc_check(nc_read_field(init_ncf_la_filename, 'LAr', c), init_ncf_la_filename)
+    BRr = c/Uw_scale
+    call nc_check(nc_read_field(init_ncf_la_filename, 'LAi', c), init_ncf_la_filename)
+    BIr = c/Uw_scale
+    call nc_check(nc_read_time(init_ncf_la_filename, t), init_ncf_la_filename)
+    write(*, '(a,es16.8,a)') '# L+A read, file time ', t, ' s'
+    deallocate(c)
+  end subroutine read_ncf_la
+
+  subroutine nc_check(code, fname)
+    integer, intent(in) :: code
+    character(len=*), intent(in) :: fname
+    if (code /= 0) then
+      write(*, '(a,a,a,a)') 'qg_driver: netCDF ', trim(fname), ': ', nc_error()
+      stop 1
+    end if
+  end subroutine nc_check
+
+  ! N^2 at the staggered levels in s^-2 (ncdump_N2, IO_ncf.f90:376-432)
+  subroutine dump_n2()
+    real(c_float), allocatable, target :: w(:)
+    allocate(w(n3))
+    w = real(N2s*N0*N0, c_float)
+    call nc_check(nc_write_profile(trim(out_prefix)//'N2.out.nc', 'N2', w), trim(out_prefix)//'N2.out.nc')
+    deallocate(w)
+  end subroutine dump_n2
+
+  ! psi and L+A in r-space after step i, as ncdump_psi / ncdump_la
+  ! (IO_ncf.f90:15-110, 116-217): FLOAT values psi U_scale L_scale and
+  ! BR, BI times Uw_scale, time i dt (default REAL, main_waqg.f90:197, 328)
+  ! times L_scale/U_scale in seconds; files numbered from 000
+  subroutine dump_fields(i)
+    integer, intent(in) :: i
     complex(c_double_complex), allocatable, target :: pk(:, :, :), brk(:, :, :), bik(:, :, :)
     real(c_double), allocatable, target :: r(:, :, :)
-    character(len=8) :: tag
-    integer :: v
-    if (len_trim(out_prefix) == 0) return
+    real(c_float), allocatable, target :: w(:, :, :, :)
+    real :: time
+    real(c_float) :: ts
+    character(len=16) :: tag
+    character(len=512) :: fname
     allocate(pk(iktx, ikty, n3), brk(iktx, ikty, n3), bik(iktx, ikty, n3), r(n1, n1, n3))
     call check(qg_get_state(ctx, c_loc(pk), c_null_ptr, c_null_ptr, c_loc(brk), c_loc(bik), c_null_ptr, &
                             c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr), 'get_state')
-    write(tag, '(i4.4)') dump
-    open(newunit=v, file=trim(out_prefix)//'psi'//trim(tag)//'.bin', access='stream', form='unformatted', &
-         status='replace', action='write')
+    time = i*delt
+    ts = real(time*L_scale/U_scale, c_float)
+    write(tag, '(i0.3)') dump
+    allocate(w(n1, n1, n3, 2))
     call check(qg_fft_c2r(ctx, c_loc(pk), c_loc(r), int(n1, c_int32_t), int(n3, c_int32_t)), 'c2r')
-    write(v) real(r, 4)
-    close(v)
-    open(newunit=v, file=trim(out_prefix)//'la'//trim(tag)//'.bin', access='stream', form='unformatted', &
-         status='replace', action='write')
+    w(:, :, :, 1) = real(r*U_scale*L_scale, c_float)
+    fname = trim(out_prefix)//'psi'//trim(tag)//'.out.nc'
+    call nc_check(nc_write_fields(fname, 'psi', w(:, :, :, 1:1), ts), fname)
     call check(qg_fft_c2r(ctx, c_loc(brk), c_loc(r), int(n1, c_int32_t), int(n3, c_int32_t)), 'c2r')
-    write(v) real(r, 4)
+    w(:, :, :, 1) = real(r*Uw_scale, c_float)
     call check(qg_fft_c2r(ctx, c_loc(bik), c_loc(r), int(n1, c_int32_t), int(n3, c_int32_t)), 'c2r')
-    write(v) real(r, 4)
-    close(v)
-    deallocate(pk, brk, bik, r)
+    w(:, :, :, 2) = real(r*Uw_scale, c_float)
+    fname = trim(out_prefix)//'la'//trim(tag)//'.out.nc'
+    call nc_check(nc_write_fields(fname, 'LAr,LAi', w, ts), fname)
+    deallocate(pk, brk, bik, r, w)
     dump = dump + 1
   end subroutine dump_fields
 

@@ -26,20 +26,33 @@ module qg_run_params
   double precision :: n2_ratio = 100.d0
   integer :: itermax = 10              ! steps to run (the first one is the Euler step)
   integer :: diag_every = 1            ! print the energy diagnostics every diag_every steps (0: never)
-  integer :: out_every = 0             ! dump psi and L+A in r-space every out_every steps (0: never)
+  integer :: out_every = 0             ! netCDF dumps of psi and L+A in r-space every out_every steps (0: never)
   character(len=256) :: ic_prefix = ' '   ! read <prefix>psi.in, BR.in, BI.in (float64 n1*n1*n3); blank: analytic
-  character(len=256) :: out_prefix = ' '  ! final spectral state to <prefix>state.bin; blank: none
+  character(len=256) :: out_prefix = ' '  ! prefix of the dumps; final spectral state to <prefix>state.bin unless blank
   ! gradient mode (no counterpart in the reference): grad_steps > 0 computes, instead of the run,
   ! the deep wave energy after grad_steps steps and its gradient with respect to the initial psi
   ! (checkpoints every grad_every steps, levels with z_s < grad_zdeep); the gradient goes to
   ! <out_prefix>grad.bin (float64 n1*n1*n3)
   integer :: grad_steps = 0, grad_every = 20
   double precision :: grad_zdeep = 0.8d0*6.28318530717958647692d0
+  ! netCDF input and output (IO_ncf.f90; main_waqg.f90:137-139, 153-155, 470-471): with
+  ! init_ncf_psi = 1 (init_ncf_la = 1) psi (L+A) is read from init_ncf_psi_filename
+  ! (init_ncf_la_filename) in place of the fields above; with out_every > 0, N2.out.nc once and
+  ! psiNNN.out.nc, laNNN.out.nc at step 0 and every out_every steps (names prefixed with
+  ! out_prefix).  Values are dimensional with the reference's scales (parameters_test1.f90:8,
+  ! 45-50: default-REAL literals as there); they scale the files only, the step takes Ro, Bu and
+  ! W2F as given.
+  integer :: init_ncf_psi = 0, init_ncf_la = 0
+  character(len=256) :: init_ncf_psi_filename = 'psi000.in.nc', init_ncf_la_filename = 'la000.in.nc'
+  double precision :: U_scale = 0.25, Uw_scale = 2.5e-5
+  double precision :: L_scale = 6.28318530718*5e4/L1     ! dom_x/L1
+  double precision :: N0 = (25./8.)*twopi*1e-4            ! (25/8) 2 pi cor
 
   namelist /qg/ n1, n3, delt, gamma, Ro, Bu, W2F, coeff1, coeff2, coeff1w, coeff2w, coeffz, &
                 ilap1, ilap2, ilap1w, ilap2w, fixed_flow, no_feedback, passive_scalar, linear, &
                 no_dispersion, inviscid, stratification, n2_ratio, itermax, diag_every, out_every, &
-                ic_prefix, out_prefix, grad_steps, grad_every, grad_zdeep
+                ic_prefix, out_prefix, grad_steps, grad_every, grad_zdeep, init_ncf_psi, init_ncf_la, &
+                init_ncf_psi_filename, init_ncf_la_filename, U_scale, Uw_scale, L_scale, N0
 
 contains
 

@@ -22,6 +22,15 @@ from qgybj.params import make_params, nuh_coeff
 
 DRIVER = os.path.join(PKG, "lib", "qg_driver")
 
+# The reference's dimensional scales (parameters_test1.f90:8, 45-50), with its
+# default-REAL literals: dom_x = 6.28318530718*5e4 is a REAL(4) product,
+# Uw_scale and cor = 1e-4 are REAL(4) literals stored in FP64.
+TWOPI = 2.0 * np.pi
+U_SCALE = 0.25
+UW_SCALE = float(np.float32(2.5e-5))
+L_SCALE = float(np.float32(6.28318530718) * np.float32(5e4)) / TWOPI
+N0 = (3.125 * TWOPI) * float(np.float32(1e-4))
+
 pytestmark = pytest.mark.skipif(shutil.which("amdflang") is None and not os.path.exists(DRIVER),
                                 reason="no Fortran compiler and no prebuilt driver")
 
@@ -67,6 +76,10 @@ def test_driver_parameters_match_harness(tmp_path, n1, c1, l1, c2, l2):
     # 17 significant digits printed: equal as FP64 numbers
     for got, w in zip(vals, want):
         assert got == w or abs(got - w) <= 1e-16 * abs(w), (vals, want)
+    head = [ln for ln in r.stdout.splitlines() if ln.startswith("# U_scale")]
+    assert head, r.stdout + r.stderr
+    vals = [float(x) for x in head[0].split()[5:]]
+    assert vals == [U_SCALE, UW_SCALE, L_SCALE, N0], vals
 
 
 @pytest.mark.gpu
@@ -132,3 +145,63 @@ def test_driver_gradient_matches_harness(tmp_path):
     m.close()
     assert J == J2, (J, J2)
     assert np.array_equal(g, g2)
+
+
+@pytest.mark.gpu
+def test_driver_netcdf_io(tmp_path):
+    """Initial conditions from netCDF files written by another implementation
+    (SciPy), dumps read back by it: the schema of IO_ncf.f90 (dimensions,
+    variables, FLOAT values in dimensional units, time in seconds), N^2, and
+    the run itself against the oracle started from the same fields."""
+    from scipy.io import netcdf_file
+    n1, n3, nsteps, every, dt = 32, 16, 10, 5, 2e-3
+    P = make_params(n1, n3, dt=dt, W2F=0.05)
+    psi, BR, BI = physical_ics(P, k0=3, seed=4)
+    files = {"psi000.in.nc": {"psi": (psi * (U_SCALE * L_SCALE)).astype(np.float32)},
+             "la000.in.nc": {"LAr": (BR * UW_SCALE).astype(np.float32), "LAi": (BI * UW_SCALE).astype(np.float32)}}
+    for name, fields in files.items():
+        with netcdf_file(os.path.join(tmp_path, name), "w", version=2) as f:
+            for d, n in (("x", n1), ("y", n1), ("z", n3), ("t", 1)):
+                f.createDimension(d, n)
+            for v, a in fields.items():
+                f.createVariable(v, "f4", ("z", "y", "x"))[:] = a
+            f.createVariable("time", "f4", ("t",))[:] = [0.0]
+    print(f"fortran driver netCDF io n1={n1} n3={n3}", flush=True)
+    r = _run(tmp_path, n1=n1, n3=n3, delt=dt, W2F=0.05, itermax=nsteps, diag_every=5, out_every=every,
+             init_ncf_psi=1, init_ncf_la=1, init_ncf_psi_filename=os.path.join(tmp_path, "psi000.in.nc"),
+             init_ncf_la_filename=os.path.join(tmp_path, "la000.in.nc"), out_prefix=os.path.join(tmp_path, "out_"))
+    assert r.returncode == 0, r.stdout + r.stderr
+
+    # the oracle from the fields the reference recovers (IO_ncf.f90:280-287, 357-366)
+    g = O.Grid(n1, n1, n3)
+    OP = oracle_params(P)
+    psi0 = files["psi000.in.nc"]["psi"].astype(np.float64) / (U_SCALE * L_SCALE)
+    BR0 = files["la000.in.nc"]["LAr"].astype(np.float64) / UW_SCALE
+    BI0 = files["la000.in.nc"]["LAi"].astype(np.float64) / UW_SCALE
+    S = O.initial_state(g, OP, psi0, BR0, BI0)
+    states = {0: S}
+    for it in range(1, nsteps + 1):
+        S = O.step(g, OP, S)
+        if it % every == 0:
+            states[it] = S
+    raw = np.fromfile(os.path.join(tmp_path, "out_state.bin"), dtype=np.complex128).reshape(len(FIELDS), n3, n1, -1)
+    compare_state({f: raw[i] for i, f in enumerate(FIELDS)}, S, 1e-11)
+
+    with netcdf_file(os.path.join(tmp_path, "out_N2.out.nc"), "r", mmap=False) as f:
+        assert f.dimensions == {"z": n3} and list(f.variables) == ["N2"]
+        np.testing.assert_allclose(f.variables["N2"].data, (P.N2s * N0 * N0).astype(np.float32), rtol=1e-6)
+    for k, it in enumerate(sorted(states)):
+        time_s = np.float32(float(np.float32(it * dt)) * L_SCALE / U_SCALE)
+        want = {f"psi{k:03d}": {"psi": O.fft_c2r(states[it].psik) * (U_SCALE * L_SCALE)},
+                f"la{k:03d}": {"LAr": O.fft_c2r(states[it].BRk) * UW_SCALE, "LAi": O.fft_c2r(states[it].BIk) * UW_SCALE}}
+        for stem, fields in want.items():
+            with netcdf_file(os.path.join(tmp_path, f"out_{stem}.out.nc"), "r", mmap=False) as f:
+                assert f.version_byte == 2
+                assert f.dimensions == {"x": n1, "y": n1, "z": n3, "t": 1}
+                assert list(f.variables) == list(fields) + ["time"]
+                assert f.variables["time"].dimensions == ("t",) and f.variables["time"].data[0] == time_s
+                for v, a in fields.items():
+                    d = f.variables[v]
+                    assert d.dimensions == ("z", "y", "x") and d.data.dtype == np.dtype(">f4")
+                    # FLOAT rounding of the FP64 fields: within a few float ulps of the largest value
+                    np.testing.assert_allclose(d.data, a.astype(np.float32), rtol=0, atol=1e-6 * np.abs(a).max())
