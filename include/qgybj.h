@@ -125,6 +125,17 @@ int qg_iter(const qg_ctx* ctx, int32_t* iter);
  * out[5] enstrophy 0.5<zeta^2> = sum kh2^2 |psi|^2; all averaged over z. */
 int qg_diag(qg_ctx* ctx, double* out);
 
+/* hspec_waves (diagnostics.f90:547-638): horizontal spectra of the waves at
+ * nlev levels (1-based, 1 <= levels[i] <= n3, as the reference's height
+ * array), from the current state in FP64 (the reference sums in REAL(4)).
+ * For shells mode = floor(|k| + 1/2) = 0 .. n1/2 - 1 of the L = 1 modes:
+ * out[((member * nlev + i) * (n1/2) + mode) * 3 + j] holds j = 0 the kinetic
+ * part 1/2 sum |BR + i BI|^2, j = 1 the potential part
+ * 1/2 sum 1/4 kh^2/Bu (|C|^2/N^2 at the interfaces below and above the level;
+ * the one above only at level 1), j = 2 the number of modes (2 per mode);
+ * nondimensional, as the reference writes them to h<i>w.dat. */
+int qg_hspec_waves(qg_ctx* ctx, const int32_t* levels, int32_t nlev, double* out);
+
 /* init_q (init.f90:591-652) on host arrays: q = T(kh2) psi / dz^2 at L = 1. */
 int qg_init_q(qg_ctx* ctx, const double* psik, double* qk);
 

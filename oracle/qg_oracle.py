@@ -21,7 +21,9 @@ need it (tests/test_oracle_*.py):
                        single- and two-mode fields;
 * time update       -- pure hyperdiffusive decay, Robert filter fixed point,
                        dispersion-only rotation rate from the analytic
-                       vertical eigenvalue.
+                       vertical eigenvalue;
+* wave spectra      -- single-mode and linear-in-z closed forms, shell counts
+                       equal to the retained-mode counts.
 
 Array convention: spectral fields are complex128 arrays of shape
 (nz, ikty, iktx) -- byte-for-byte the reference's Fortran ``(iktx,ikty,nz)``
@@ -649,6 +651,41 @@ def diagnostics(grid: Grid, P: Params, S: State) -> np.ndarray:
     WCE = np.sum(cp) / n3
     ENS = np.sum(lsum(kh2 * kh2 * ab2)) / n3
     return np.array([KE, PE, WKE, WPE, WCE, ENS])
+
+
+def hspec_waves(grid: Grid, P: Params, S: State, height: int) -> np.ndarray:
+    """hspec_waves (diagnostics.f90:547-638) at the 1-based level ``height``,
+    summed in FP64 (the reference sums in REAL(4)): rows mode = 0 .. ktx-1
+    (ktx = n1/2) of [KE, PE, number of modes] over the L = 1 modes with
+    mode = ifix(real(kh L1/twopi + 0.5)) (:605-607),
+      KE += |BR + i BI|^2 (:611),
+      PE += 1/4 kh2 (|C_{k-1}|^2/(Bu N2u_{k-1}) + |C_k|^2/(Bu N2u_k)),
+            only the k term at level 1 (:613-619); C = A_z from the state,
+      modes += 2 (:622),
+    then KE, PE halved (:627).  r_2 at izh2s is N^2 at the unstaggered level
+    of the same index (r_2(izh2) = N^2(zah2(izh2)), init.f90:180-199)."""
+    ktx = grid.n1 // 2
+    z = height - 1
+    kh2 = grid.kh2.astype(np.float64)
+    kh = np.sqrt(kh2)
+    mode = np.float32(kh * TWOPI / TWOPI + 0.5).astype(np.int64)
+    L = grid.Lb
+    dz = grid.dz
+    CR = np.zeros_like(S.ARk)
+    CI = np.zeros_like(S.AIk)
+    CR[:-1] = (S.ARk[1:] - S.ARk[:-1]) / dz
+    CI[:-1] = (S.AIk[1:] - S.AIk[:-1]) / dz
+    ke = np.abs(S.BRk[z] + 1j * S.BIk[z]) ** 2
+
+    def cpart(k):
+        return (1.0 / (P.Bu * P.N2u[k])) * kh2 * (np.abs(CR[k]) ** 2 + np.abs(CI[k]) ** 2)
+
+    pe = 0.25 * (cpart(z) if z == 0 else cpart(z - 1) + cpart(z))
+    out = np.zeros((ktx + 1, 3))
+    for col, val in ((0, ke), (1, pe), (2, np.full(ke.shape, 2.0))):
+        np.add.at(out[:, col], mode[L], val[L])
+    out[:, :2] *= 0.5
+    return out[:ktx]
 
 
 def rel_l2(a, b, grid: Grid | None = None) -> float:

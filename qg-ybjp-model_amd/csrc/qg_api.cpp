@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -534,6 +535,35 @@ int qg_diag(qg_ctx* c, double* out) {
       for (int b = 0; b < nb; ++b) s += part[((size_t)m * nb + b) * 6 + j];
       out[m * 6 + j] = s / c->d.g.n3;
     }
+  return QG_OK;
+}
+
+int qg_hspec_waves(qg_ctx* c, const int32_t* levels, int32_t nlev, double* out) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!levels || !out || nlev < 1) return fail(c, QG_ERR_ARG, "bad spectra arguments");
+  const Geo& g = c->d.g;
+  for (int i = 0; i < nlev; ++i)
+    if (levels[i] < 1 || levels[i] > g.n3) return fail(c, QG_ERR_ARG, "spectrum level outside 1 .. n3");
+  const int nbin = g.n1 / 2, nm = g.nm;
+  const size_t per = (size_t)nbin * 3;
+  double* dout = nullptr;
+  HIPCHK(c, hipMalloc(&dout, (size_t)nm * qg::SpecLevels::MAXL * per * sizeof(double)));
+  std::vector<double> h((size_t)nm * qg::SpecLevels::MAXL * per);
+  hipError_t e = hipSuccess;
+  for (int i0 = 0; i0 < nlev && e == hipSuccess; i0 += qg::SpecLevels::MAXL) {
+    qg::SpecLevels L;
+    L.n = std::min(nlev - i0, qg::SpecLevels::MAXL);
+    for (int i = 0; i < L.n; ++i) L.lev[i] = levels[i0 + i];
+    e = qg::launch_hspec_waves(c->d, c->stream, L, nbin, dout);
+    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), dout, (size_t)nm * L.n * per * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    for (int m = 0; e == hipSuccess && m < nm; ++m)
+      std::copy(h.begin() + (size_t)m * L.n * per, h.begin() + (size_t)(m + 1) * L.n * per,
+                out + ((size_t)m * nlev + i0) * per);
+  }
+  hipFree(dout);
+  HIPCHK(c, e);
   return QG_OK;
 }
 

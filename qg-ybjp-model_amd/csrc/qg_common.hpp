@@ -127,6 +127,14 @@ hipError_t launch_clear_outside(const Dev& d, hipStream_t st, double2* f, int al
 hipError_t launch_init_q(const Dev& d, hipStream_t st, const double2* psi, double2* q);
 hipError_t launch_cz(const Dev& d, hipStream_t st, const double2* A, double2* C);
 hipError_t launch_diag(const Dev& d, hipStream_t st, int* nblocks);
+// levels (1-based, as the reference's height array) of one spectra launch
+struct SpecLevels {
+  static constexpr int MAXL = 32;
+  int n;
+  int lev[MAXL];
+};
+// out[((member * L.n + level) * nbin + shell) * 3 + {KE, PE, modes}]
+hipError_t launch_hspec_waves(const Dev& d, hipStream_t st, const SpecLevels& L, int nbin, double* out);
 hipError_t launch_r2c(const Dev& d, hipStream_t st, const double* r, int ld, double2* k, int nz);
 hipError_t launch_c2r(const Dev& d, hipStream_t st, const double2* k, double* r, int ld, int nz);
 int diag_blocks(const Geo& g);

@@ -460,6 +460,89 @@ __global__ __launch_bounds__(NT) void k_diag(Dev d) {
 }
 
 // ---------------------------------------------------------------------------
+// hspec_waves (diagnostics.f90:547-638): horizontal spectra of the waves at
+// one level.  The L = 1 modes fall into shells mode = floor(|k| + 1/2); per
+// shell
+//   KE = 1/2 sum |BR + i BI|^2
+//   PE = 1/2 sum 1/4 kh^2 / Bu (|C_{k-1}|^2 / N^2_{k-1} + |C_k|^2 / N^2_k)
+//        with C = A_z at the interfaces below and above the level (only the
+//        one above at the bottom level; C = 0 at the top)
+//   modes = 2 per mode.
+// One workgroup per (level, member).  Each round stages one mode per thread
+// in LDS; thread t then sums the staged modes of shells t and t + NSP in
+// staging order, so the sums do not depend on scheduling.
+// ---------------------------------------------------------------------------
+constexpr int NSP = 256;
+
+__global__ __launch_bounds__(NSP) void k_hspec_waves(Dev d, SpecLevels L, int nbin, double* __restrict__ out) {
+  const Geo& g = d.g;
+  const int li = blockIdx.x, m = blockIdx.y, t = threadIdx.x;
+  const int z = L.lev[li] - 1;
+  const size_t plane = (size_t)g.ikty * g.sp;
+  const size_t mz = (size_t)m * g.n3;
+  const int per = g.KX * g.nbox;
+  const double Bu = d.Bu, dz = d.dz;
+  __shared__ int cb[NSP];
+  __shared__ double cke[NSP], cpe[NSP];
+  double ke[2] = {0.0, 0.0}, pe[2] = {0.0, 0.0}, nm[2] = {0.0, 0.0};
+  for (int base = 0; base < per; base += NSP) {
+    const int r = base + t;
+    int bin = -1;
+    double k1 = 0.0, p1 = 0.0;
+    if (r < per) {
+      const int kx = r % g.KX, iky = box_row(r / g.KX, g);
+      if (lmask(kx, iky, g)) {
+        const int ky = ky_of(iky, g.n2);
+        const double kh2 = (double)(kx * kx + ky * ky);
+        bin = (int)floor(sqrt(kh2) + 0.5);
+        const size_t col = (size_t)iky * g.sp + kx;
+        const double2 br = d.BR[(mz + z) * plane + col], bi = d.BI[(mz + z) * plane + col];
+        const double lr = br.x - bi.y, lim = br.y + bi.x;  // BR + i BI
+        k1 = lr * lr + lim * lim;
+        // |C_k|^2 kh^2 / (Bu N^2_k), C_k = (A_{k+1} - A_k)/dz, 0 at the top interface
+        double c[2] = {0.0, 0.0};
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+          const int k = z - 1 + s;
+          if (k >= 0 && k < g.n3 - 1) {
+            const size_t i0 = (mz + k) * plane + col, i1 = i0 + plane;
+            const double2 a0 = d.AR[i0], a1 = d.AR[i1], b0 = d.AI[i0], b1 = d.AI[i1];
+            const double2 cr = make_double2((a1.x - a0.x) / dz, (a1.y - a0.y) / dz);
+            const double2 ci = make_double2((b1.x - b0.x) / dz, (b1.y - b0.y) / dz);
+            c[s] = (1.0 / (Bu * d.N2u[k])) * kh2 * (cr.x * cr.x + cr.y * cr.y + ci.x * ci.x + ci.y * ci.y);
+          }
+        }
+        p1 = 0.25 * (z == 0 ? c[1] : c[0] + c[1]);
+      }
+    }
+    cb[t] = bin;
+    cke[t] = k1;
+    cpe[t] = p1;
+    __syncthreads();
+    for (int j = 0; j < NSP; ++j) {
+      const int b = cb[j] - t;
+      if (b == 0 || b == NSP) {
+        const int h = b == 0 ? 0 : 1;
+        ke[h] += cke[j];
+        pe[h] += cpe[j];
+        nm[h] += 2.0;
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int b = t + h * NSP;
+    if (b < nbin) {
+      double* o = out + (((size_t)m * L.n + li) * nbin + b) * 3;
+      o[0] = 0.5 * ke[h];
+      o[1] = 0.5 * pe[h];
+      o[2] = nm[h];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // raw transforms (fft.f90:55-83) for initial conditions and tests
 // ---------------------------------------------------------------------------
 // r2c in x of pairs of real rows -> half rows (all kx), scaled
@@ -655,6 +738,12 @@ hipError_t launch_diag(const Dev& d, hipStream_t st, int* nblocks) {
   const int nb = diag_blocks(d.g);
   *nblocks = nb;
   hipLaunchKernelGGL(k_diag, dim3(nb, d.g.nm), dim3(NT), 0, st, d);
+  return hipGetLastError();
+}
+
+hipError_t launch_hspec_waves(const Dev& d, hipStream_t st, const SpecLevels& L, int nbin, double* out) {
+  if (L.n < 1 || L.n > SpecLevels::MAXL || nbin < 1 || nbin > 2 * NSP) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_hspec_waves, dim3(L.n, d.g.nm), dim3(NSP), 0, st, d, L, nbin, out);
   return hipGetLastError();
 }
 

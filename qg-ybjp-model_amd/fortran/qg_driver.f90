@@ -25,6 +25,7 @@ program qg_driver
   implicit none
 
   integer :: iktx, ikty, ix, iy, iz, it, nxt, u, ios, dump
+  integer :: hunit(10)
   integer(c_int) :: rc
   integer(c_int32_t) :: iter
   character(len=512) :: arg
@@ -117,7 +118,9 @@ program qg_driver
   deallocate(psir, BRr, BIr)
 
   dump = 0
+  if (hspec_every > 0) call open_spectra()
   if (diag_every > 0) call report(0)
+  if (hspec_every > 0) call spectra()
   if (out_every > 0) call dump_n2()
   if (out_every > 0) call dump_fields(0)
 
@@ -127,6 +130,7 @@ program qg_driver
   do while (it < itermax)
     nxt = itermax
     if (diag_every > 0) nxt = min(nxt, (it/diag_every + 1)*diag_every)
+    if (hspec_every > 0) nxt = min(nxt, (it/hspec_every + 1)*hspec_every)
     if (out_every > 0) nxt = min(nxt, (it/out_every + 1)*out_every)
     call check(qg_step(ctx, int(nxt - it, c_int32_t)), 'step')
     it = nxt
@@ -136,7 +140,15 @@ program qg_driver
     if (out_every > 0) then
       if (mod(it, out_every) == 0) call dump_fields(it)
     end if
+    if (hspec_every > 0) then
+      if (mod(it, hspec_every) == 0) call spectra()
+    end if
   end do
+  if (hspec_every > 0) then
+    do iz = 1, 10
+      close(hunit(iz))
+    end do
+  end if
 
   if (len_trim(out_prefix) > 0) then
     allocate(fk(iktx, ikty, n3, 9))
@@ -251,6 +263,38 @@ contains
       stop 1
     end if
   end subroutine nc_check
+
+  ! h0w.dat .. h9w.dat (files.f90:182-191)
+  subroutine open_spectra()
+    character(len=8) :: nm
+    integer :: k
+    do k = 1, 10
+      write(nm, '(a,i0,a)') 'h', k - 1, 'w.dat'
+      open(newunit=hunit(k), file=trim(out_prefix)//trim(nm), action='write', status='replace')
+    end do
+  end subroutine open_spectra
+
+  ! hspec_waves at the heights of parameters_test1.f90:188 (diagnostics.f90:
+  ! 625-634): per level one row per populated shell (mode, KE, PE, number of
+  ! modes, as default REAL) and a blank line
+  subroutine spectra()
+    integer(c_int32_t) :: hs(10)
+    real(c_double), allocatable :: sp(:, :, :)
+    integer :: k, mode
+    hs = int([1, n3/8, n3/4, 3*n3/8, n3/2, 5*n3/8, 3*n3/4, 7*n3/8, n3 - 2, n3], c_int32_t)
+    hs = max(hs, 1_c_int32_t)
+    allocate(sp(3, n1/2, 10))
+    call check(qg_hspec_waves(ctx, hs, 10_c_int32_t, sp), 'hspec_waves')
+    do k = 1, 10
+      do mode = 0, n1/2 - 1
+        if (sp(3, mode + 1, k) /= 0.d0) write(hunit(k), fmt=*) real(mode), real(sp(1, mode + 1, k)), &
+                                                                real(sp(2, mode + 1, k)), real(sp(3, mode + 1, k))
+      end do
+      write(hunit(k), *) '           '
+      flush(hunit(k))
+    end do
+    deallocate(sp)
+  end subroutine spectra
 
   ! N^2 at the staggered levels in s^-2 (ncdump_N2, IO_ncf.f90:376-432)
   subroutine dump_n2()

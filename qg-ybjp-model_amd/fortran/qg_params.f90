@@ -42,6 +42,10 @@ module qg_run_params
   ! out_prefix).  Values are dimensional with the reference's scales (parameters_test1.f90:8,
   ! 45-50: default-REAL literals as there); they scale the files only, the step takes Ro, Bu and
   ! W2F as given.
+  ! wave spectra (hspec_waves, diagnostics.f90:547-638) at the reference's 10 heights
+  ! (parameters_test1.f90:188) to <out_prefix>h0w.dat .. h9w.dat at step 0 and every hspec_every
+  ! steps (0: never)
+  integer :: hspec_every = 0
   integer :: init_ncf_psi = 0, init_ncf_la = 0
   character(len=256) :: init_ncf_psi_filename = 'psi000.in.nc', init_ncf_la_filename = 'la000.in.nc'
   double precision :: U_scale = 0.25, Uw_scale = 2.5e-5
@@ -52,7 +56,7 @@ module qg_run_params
                 ilap1, ilap2, ilap1w, ilap2w, fixed_flow, no_feedback, passive_scalar, linear, &
                 no_dispersion, inviscid, stratification, n2_ratio, itermax, diag_every, out_every, &
                 ic_prefix, out_prefix, grad_steps, grad_every, grad_zdeep, init_ncf_psi, init_ncf_la, &
-                init_ncf_psi_filename, init_ncf_la_filename, U_scale, Uw_scale, L_scale, N0
+                init_ncf_psi_filename, init_ncf_la_filename, U_scale, Uw_scale, L_scale, N0, hspec_every
 
 contains
 

@@ -76,6 +76,15 @@ module qgybj_c
       real(c_double), intent(out) :: out(6)
     end function
 
+    ! wave spectra at nlev 1-based levels: out(3, n1/2, nlev) = KE, PE, modes per shell
+    integer(c_int) function qg_hspec_waves(ctx, levels, nlev, out) bind(C, name="qg_hspec_waves")
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), intent(in) :: levels(*)
+      integer(c_int32_t), value :: nlev
+      real(c_double), intent(out) :: out(*)
+    end function
+
     integer(c_int) function qg_fft_c2r(ctx, k, r, ldx, nz) bind(C, name="qg_fft_c2r")
       import :: c_int, c_ptr, c_int32_t
       type(c_ptr), value :: ctx, k, r

@@ -28,7 +28,7 @@ KERNEL_NAMES = ["col_inv", "row_nl", "col_upd", "row_qw", "col_qw", "solve", "so
 # exported symbols declared in include/qgybj.h
 SYMBOLS = [
     "qg_create", "qg_create_batch", "qg_members", "qg_destroy", "qg_last_error", "qg_set_state", "qg_set_state_physical", "qg_get_state",
-    "qg_step", "qg_iter", "qg_diag", "qg_init_q", "qg_fft_r2c", "qg_fft_c2r", "qg_device_field", "qg_device_pitch", "qg_stream",
+    "qg_step", "qg_iter", "qg_diag", "qg_hspec_waves", "qg_init_q", "qg_fft_r2c", "qg_fft_c2r", "qg_device_field", "qg_device_pitch", "qg_stream",
     "qg_sync", "qg_timing", "qg_timing_sample", "qg_kernel_times", "qg_kernel_name", "qg_set_variants", "qg_get_variants",
     "qg_ensemble_create", "qg_ensemble_destroy", "qg_ensemble_error", "qg_ensemble_set_graph_steps",
     "qg_ensemble_step", "qg_tune", "qg_adj_create", "qg_adj_destroy", "qg_adj_error", "qg_adj_set_fused", "qg_adj_objective",
@@ -77,6 +77,7 @@ def load(path: str = LIB_PATH):
     lib.qg_step.argtypes = [vp, C.c_int32]
     lib.qg_iter.argtypes = [vp, C.POINTER(C.c_int32)]
     lib.qg_diag.argtypes = [vp, dp]
+    lib.qg_hspec_waves.argtypes = [vp, C.POINTER(C.c_int32), C.c_int32, dp]
     lib.qg_init_q.argtypes = [vp, dp, dp]
     lib.qg_fft_r2c.argtypes = [vp, dp, C.c_int32, dp, C.c_int32]
     lib.qg_fft_c2r.argtypes = [vp, dp, dp, C.c_int32, C.c_int32]
@@ -219,6 +220,16 @@ class Member:
         """[KE, PE, WKE, WPE, WCE, ENS] (per member: shape (members, 6))."""
         out = np.zeros((self.members, 6))
         self._chk(self.lib.qg_diag(self._h, _dp(out)), "qg_diag")
+        return out[0] if self.members == 1 else out
+
+    def hspec_waves(self, levels) -> np.ndarray:
+        """Wave spectra at the given 1-based levels (diagnostics.f90:547-638):
+        shape (len(levels), n1/2, 3) = [KE, PE, number of modes] per shell
+        (a leading members axis for a batch)."""
+        lv = np.ascontiguousarray(levels, dtype=np.int32)
+        out = np.zeros((self.members, len(lv), self.n1 // 2, 3))
+        self._chk(self.lib.qg_hspec_waves(self._h, lv.ctypes.data_as(C.POINTER(C.c_int32)), len(lv),
+                                          _dp(out)), "qg_hspec_waves")
         return out[0] if self.members == 1 else out
 
     def init_q(self, psik):

@@ -169,7 +169,8 @@ def test_driver_netcdf_io(tmp_path):
     print(f"fortran driver netCDF io n1={n1} n3={n3}", flush=True)
     r = _run(tmp_path, n1=n1, n3=n3, delt=dt, W2F=0.05, itermax=nsteps, diag_every=5, out_every=every,
              init_ncf_psi=1, init_ncf_la=1, init_ncf_psi_filename=os.path.join(tmp_path, "psi000.in.nc"),
-             init_ncf_la_filename=os.path.join(tmp_path, "la000.in.nc"), out_prefix=os.path.join(tmp_path, "out_"))
+             init_ncf_la_filename=os.path.join(tmp_path, "la000.in.nc"), out_prefix=os.path.join(tmp_path, "out_"),
+             hspec_every=every)
     assert r.returncode == 0, r.stdout + r.stderr
 
     # the oracle from the fields the reference recovers (IO_ncf.f90:280-287, 357-366)
@@ -205,3 +206,23 @@ def test_driver_netcdf_io(tmp_path):
                     assert d.dimensions == ("z", "y", "x") and d.data.dtype == np.dtype(">f4")
                     # FLOAT rounding of the FP64 fields: within a few float ulps of the largest value
                     np.testing.assert_allclose(d.data, a.astype(np.float32), rtol=0, atol=1e-6 * np.abs(a).max())
+
+    # wave spectra at the reference's heights (parameters_test1.f90:188): one
+    # block per call (steps 0, 5, 10), rows of populated shells
+    heights = [1, n3 // 8, n3 // 4, 3 * n3 // 8, n3 // 2, 5 * n3 // 8, 3 * n3 // 4, 7 * n3 // 8, n3 - 2, n3]
+    for i, h in enumerate(heights):
+        blocks, cur = [], []
+        for ln in open(os.path.join(tmp_path, f"out_h{i}w.dat")):
+            if ln.strip():
+                cur.append([float(x) for x in ln.split()])
+            else:
+                blocks.append(np.array(cur))
+                cur = []
+        assert len(blocks) == len(states) and not cur, (i, len(blocks))
+        for got, it in zip(blocks, sorted(states)):
+            want = O.hspec_waves(g, OP, states[it], h)
+            rows = np.nonzero(want[:, 2])[0]
+            assert np.array_equal(got[:, 0], rows) and np.array_equal(got[:, 3], want[rows, 2])
+            # written as default REAL, like the reference (deep levels of surface-trapped waves underflow)
+            w32 = want[rows, :2].astype(np.float32).astype(np.float64)
+            np.testing.assert_allclose(got[:, 1:3], w32, rtol=2e-6, atol=1e-6 * np.abs(w32).max() + 1e-44)

@@ -1,0 +1,69 @@
+"""Device wave spectra (qg_hspec_waves, diagnostics.f90:547-638) through the
+C ABI against the oracle's restatement, at the reference's spectrum heights
+(parameters_test1.f90:188): shell counts exactly, the kinetic and potential
+parts to 1e-12; deterministic (bitwise equal on repeat); batch members equal
+single members bit for bit; more levels than one launch takes; bad levels
+rejected."""
+import numpy as np
+import pytest
+
+import qgybj
+from helpers import oracle_params, physical_ics
+from oracle import qg_oracle as O
+from qgybj.params import make_params
+
+pytestmark = pytest.mark.gpu
+
+
+def _heights(n3):
+    return [1, n3 // 8, n3 // 4, 3 * n3 // 8, n3 // 2, 5 * n3 // 8, 3 * n3 // 4, 7 * n3 // 8, n3 - 2, n3]
+
+
+def _oracle_state(st):
+    z = np.zeros_like(st["psik"])
+    return O.State(psik=st["psik"], qk=st["qk"], qok=st["qok"], BRk=st["BRk"], BIk=st["BIk"], BRok=st["BRok"],
+                   BIok=st["BIok"], ARk=st["ARk"], AIk=st["AIk"], CRk=z, CIk=z)
+
+
+@pytest.mark.parametrize("n1,n3", [(64, 32), (128, 16)])
+def test_hspec_waves_matches_oracle(n1, n3):
+    P = make_params(n1, n3, dt=2e-3, W2F=0.05, stratification="exponential")
+    psi, BR, BI = physical_ics(P, k0=3, seed=5)
+    m = qgybj.Member(P)
+    m.set_state_physical(psi, BR, BI)
+    m.step(7)
+    hs = _heights(n3)
+    got = m.hspec_waves(hs)
+    assert got.shape == (len(hs), n1 // 2, 3)
+    g = O.Grid(n1, n1, n3)
+    OP = oracle_params(P)
+    S = _oracle_state(m.get_state())
+    for i, h in enumerate(hs):
+        want = O.hspec_waves(g, OP, S, h)
+        assert np.array_equal(got[i, :, 2], want[:, 2]), h
+        for j in range(2):
+            np.testing.assert_allclose(got[i, :, j], want[:, j], rtol=1e-12, atol=1e-14 * np.abs(want[:, j]).max())
+        assert want[:, 1].max() > 0 and want[:, 0].max() > 0
+    assert np.array_equal(m.hspec_waves(hs), got)
+    for bad in ([0], [n3 + 1], []):
+        with pytest.raises(qgybj.QGError):
+            m.hspec_waves(bad)
+    m.close()
+
+
+def test_hspec_waves_batch_and_many_levels():
+    P = make_params(32, 40, dt=2e-3, W2F=0.05)
+    ics = [physical_ics(P, k0=3, seed=s) for s in (7, 8)]
+    b = qgybj.Member(P, members=2)
+    b.set_state_physical(*[np.stack([x[i] for x in ics]) for i in range(3)])
+    b.step(5)
+    levels = list(range(1, 41))  # two launches of at most 32 levels
+    gb = b.hspec_waves(levels)
+    assert gb.shape == (2, 40, 16, 3)
+    for j, ic in enumerate(ics):
+        one = qgybj.Member(P)
+        one.set_state_physical(*ic)
+        one.step(5)
+        assert np.array_equal(one.hspec_waves(levels), gb[j])
+        one.close()
+    b.close()
