@@ -44,14 +44,18 @@ enum {
 
 /* special-case switches; same meaning as the reference's integer parameters
  * fixed_flow, no_feedback, passive_scalar, linear, no_dispersion, inviscid
- * (parameters_test1.f90:22-29).  ybj_plus = 1 is implied. */
+ * (parameters_test1.f90:22-29).  QG_YBJ_REGULAR is ybj_plus = 0: the regular
+ * YBJ operator (sumB, compute_sigma, compute_A, derivatives.f90:1037-1231;
+ * main_waqg.f90:298-310, 449-461) instead of the YBJ+ inversion
+ * A_solver_ybj_plus; without it ybj_plus = 1.  Other bits are rejected. */
 enum {
   QG_FIXED_FLOW = 1,
   QG_NO_FEEDBACK = 2,
   QG_PASSIVE_SCALAR = 4,
   QG_LINEAR = 8,
   QG_NO_DISPERSION = 16,
-  QG_INVISCID = 32
+  QG_INVISCID = 32,
+  QG_YBJ_REGULAR = 64
 };
 
 /* Runtime form of the reference's compile-time parameters module

@@ -60,6 +60,8 @@ PASSIVE_SCALAR = 4
 LINEAR = 8
 NO_DISPERSION = 16
 INVISCID = 32
+# ybj_plus = 0 (main_waqg.f90:298-310, 449-461): the regular YBJ operator
+YBJ_REGULAR = 64
 
 
 # ----------------------------------------------------------------------------
@@ -398,6 +400,57 @@ def psi_solver(grid: Grid, a_ell_ut, qs):
     return _solve_columns(grid, a_ell_ut, qs, 1.0, dz * dz)
 
 
+def sumB(grid: Grid, BRk, BIk):
+    """derivatives.f90:1072-1118: subtract the vertical mean of B from every
+    L = 1 mode with kh2 > 0 (returns new arrays)."""
+    sel = grid.Lb & (grid.kh2 > 0)
+    n3 = BRk.shape[0]
+    out = []
+    for b in (BRk, BIk):
+        ave = np.sum(b, axis=0) / n3
+        out.append(np.where(sel, b - ave, b))
+    return out[0], out[1]
+
+
+def compute_sigma(grid: Grid, Bu: float, Ro: float, nBRk, nBIk, rBRk, rBIk):
+    """derivatives.f90:1037-1067: the vertical sums Bu Ro sum_z (rBR +
+    2 nBI)/kh2 and Bu Ro sum_z (rBI - 2 nBR)/kh2 at the L = 1 modes with
+    kh2 > 0 (0 elsewhere)."""
+    sel = grid.Lb & (grid.kh2 > 0)
+    kh2 = np.where(sel, grid.kh2, 1).astype(np.float64)
+    s1 = np.zeros(grid.kh2.shape, np.complex128)
+    s2 = np.zeros_like(s1)
+    for z in range(nBRk.shape[0]):
+        s1 = s1 + np.where(sel, (rBRk[z] + 2 * nBIk[z]) / kh2, 0)
+        s2 = s2 + np.where(sel, (rBIk[z] - 2 * nBRk[z]) / kh2, 0)
+    return s1 * Bu * Ro, s2 * Bu * Ro
+
+
+def compute_A(grid: Grid, N2u, BRk, BIk, sigma):
+    """derivatives.f90:1120-1231: A from the vertical integral of the
+    (mean-free) B: A~_1 = 0, A~_k = A~_{k-1} + (sum_{j<k} B_j) N^2_u(k-1) dz^2,
+    then A = A~ + (sigma - sum_k A~_k)/n3 so that sum_z A = sigma, at the L = 1
+    modes with kh2 > 0 (0 elsewhere); C = A_z (forward difference, 0 at the
+    top).  Returns AR, AI, CR, CI."""
+    sel = grid.Lb & (grid.kh2 > 0)
+    n3 = BRk.shape[0]
+    dz = grid.dz
+    out = []
+    for b, sg in ((BRk, sigma[0]), (BIk, sigma[1])):
+        at = np.zeros_like(b)
+        sb = b[0].copy()
+        sa = np.zeros_like(sb)
+        for iz in range(1, n3):
+            at[iz] = at[iz - 1] + sb * N2u[iz - 1] * dz * dz
+            sa = sa + at[iz]
+            sb = sb + b[iz]
+        a = np.where(sel, at + (sg - sa) / n3, 0)
+        c = np.zeros_like(a)
+        c[:-1] = (a[1:] - a[:-1]) / dz
+        out += [a, c]
+    return out[0], out[2], out[1], out[3]
+
+
 def A_solver_ybj_plus(grid: Grid, a_ell_ut, Bu: float, bk):
     """elliptic.f90:346-460: T'(kh2) A = dz^2 Bu B with kh2 dz^2/4 on the
     diagonal; C = A_z (forward difference, 0 at the top)."""
@@ -512,9 +565,10 @@ def _tendencies(grid: Grid, P: Params, S: State, qo_for_dq):
     return nqk, nBRk, nBIk, rBRk, rBIk, dqk, ARk, AIk
 
 
-def _invert(grid: Grid, P: Params, S: State):
+def _invert(grid: Grid, P: Params, S: State, tend):
     """q -> psi and B -> A inversions after the update (main_waqg.f90:271-311
-    / :422-462)."""
+    / :422-462); tend = (nBR, nBI, rBR, rBI) of the step for the regular YBJ
+    operator."""
     if not (P.flags & FIXED_FLOW):
         if P.flags & NO_FEEDBACK:
             qwk = np.zeros_like(S.qk)
@@ -523,8 +577,13 @@ def _invert(grid: Grid, P: Params, S: State):
         qwk = np.where(grid.Lb, S.qk - qwk, qwk)
         S.psik = psi_solver(grid, P.a_ell_ut, qwk)
     if not (P.flags & PASSIVE_SCALAR):
-        S.ARk, S.CRk = A_solver_ybj_plus(grid, P.a_ell_ut, P.Bu, S.BRk)
-        S.AIk, S.CIk = A_solver_ybj_plus(grid, P.a_ell_ut, P.Bu, S.BIk)
+        if P.flags & YBJ_REGULAR:
+            S.BRk, S.BIk = sumB(grid, S.BRk, S.BIk)
+            sigma = compute_sigma(grid, P.Bu, P.Ro, *tend)
+            S.ARk, S.AIk, S.CRk, S.CIk = compute_A(grid, P.N2u, S.BRk, S.BIk, sigma)
+        else:
+            S.ARk, S.CRk = A_solver_ybj_plus(grid, P.a_ell_ut, P.Bu, S.BRk)
+            S.AIk, S.CIk = A_solver_ybj_plus(grid, P.a_ell_ut, P.Bu, S.BIk)
 
 
 def euler_step(grid: Grid, P: Params, S: State) -> State:
@@ -551,7 +610,7 @@ def euler_step(grid: Grid, P: Params, S: State) -> State:
     S.qk = np.where(L, q1, 0).astype(np.complex128)
     S.BRk = np.where(L, BR1, 0).astype(np.complex128)
     S.BIk = np.where(L, BI1, 0).astype(np.complex128)
-    _invert(grid, P, S)
+    _invert(grid, P, S, (nBRk, nBIk, rBRk, rBIk))
     S.iter = 1
     return S
 
@@ -587,7 +646,7 @@ def leapfrog_step(grid: Grid, P: Params, S: State) -> State:
     S.qk = qt.astype(np.complex128)
     S.BRk = BRt.astype(np.complex128)
     S.BIk = BIt.astype(np.complex128)
-    _invert(grid, P, S)
+    _invert(grid, P, S, (nBRk, nBIk, rBRk, rBIk))
     S.iter += 1
     return S
 

@@ -170,6 +170,9 @@ int step_once(qg_ctx* c) {
     HIPCHK(c, hipMemcpyAsync(d.BIo, d.BI, bytes, hipMemcpyDeviceToDevice, st));
   }
   const bool full = euler || !c->p_valid;
+  // regular YBJ (ybj_plus = 0) with dispersion: A from sumB / compute_sigma /
+  // compute_A instead of the YBJ+ inversion
+  const bool regular = (flags & qg::F_YBJ_REGULAR) && !(flags & qg::F_PASSIVE);
   const int W = full ? g.iktx : g.KX;
   const int box = full ? 0 : 1;
   // u, v, zeta ingredients from psi (compute_velo derivatives.f90:1302-1311,
@@ -184,6 +187,12 @@ int step_once(qg_ctx* c) {
       jobs.j[2] = qg::ColInvJob{d.BR, {d.I[2], nullptr, nullptr}, W, 0, 0, g.nz};
       jobs.j[3] = qg::ColInvJob{d.BI, {d.I[4], nullptr, nullptr}, W, 0, 0, g.nz};
       jobs.n = 4;
+    } else if (regular) {
+      // sumB changed B after the update formed these planes: form them again
+      // from the mean-free B (the same masked box transform the update uses)
+      jobs.j[1] = qg::ColInvJob{d.BR, {d.I[2], nullptr, nullptr}, W, 1, 1, g.nz};
+      jobs.j[2] = qg::ColInvJob{d.BI, {d.I[4], nullptr, nullptr}, W, 1, 1, g.nz};
+      jobs.n = 3;
     }
     KRUN(c, qg::K_COL_INV, qg::launch_col_inv(d, st, jobs));
   }
@@ -193,7 +202,7 @@ int step_once(qg_ctx* c) {
   if (c->evA_live) HIPCHK(c, hipStreamWaitEvent(st, c->evA, 0));
   KRUN(c, qg::K_COL_UPD, qg::launch_col_upd(d, st, euler ? 1 : 0, has_dq));
   c->p_valid = 1;
-  const bool solve_a = !(flags & qg::F_PASSIVE);
+  const bool solve_a = !(flags & qg::F_PASSIVE) && !regular;
   const bool a_side = solve_a && !d.tune.a_main;
   if (a_side) {
     // A^{n+1} from B^{n+1} (A_solver_ybj_plus) on the side stream; it is
@@ -219,6 +228,8 @@ int step_once(qg_ctx* c) {
   } else if (f_hi > 0) {
     KRUN(c, qg::K_SOLVE_A, qg::launch_solve(d, st, f_lo, f_hi, nullptr));
   }
+  // after compute_qw has used B^{n+1} (main_waqg.f90:449-461)
+  if (regular) KRUN(c, qg::K_SOLVE_A, qg::launch_ybj_regular(d, st));
   if (c->clear_pending > 0) {
     if (c->evA_live) HIPCHK(c, hipStreamWaitEvent(st, c->evA, 0));
     // modes outside the dealiasing box are zero in the reference after the
@@ -311,6 +322,7 @@ int qg_create_batch(const qg_params* p, int32_t members, qg_ctx** out) {
   if (members < 1 || (int64_t)members * p->n3 > (1 << 20)) return QG_ERR_ARG;
   if (!p->ifac_q || !p->ifac_w || !p->N2_u || !p->N2_s) return QG_ERR_ARG;
   if (!(p->dt > 0.0) || !std::isfinite(p->gamma) || !(p->Ro > 0.0) || !(p->Bu > 0.0)) return QG_ERR_ARG;
+  if (p->flags & ~qg::F_ALL) return QG_ERR_ARG;
   qg_ctx* c = new qg_ctx();
   int rc;
   hipError_t e = hipGetDevice(&c->device);
@@ -354,6 +366,10 @@ int qg_create_batch(const qg_params* p, int32_t members, qg_ctx** out) {
     if ((rc = dalloc(c, f, S)) != QG_OK) { qg_destroy(c); return rc; }
   for (int i = 0; i < 6; ++i)
     if ((rc = dalloc(c, &d.I[i], (size_t)g.nz * g.n2 * g.sp)) != QG_OK) { qg_destroy(c); return rc; }
+  if (d.flags & qg::F_YBJ_REGULAR) {
+    if ((rc = dalloc(c, &d.tR, S)) != QG_OK) { qg_destroy(c); return rc; }
+    if ((rc = dalloc(c, &d.tI, S)) != QG_OK) { qg_destroy(c); return rc; }
+  }
   const size_t ncols = (size_t)g.KX * g.nbox;
   if ((rc = dalloc(c, &d.scratch, 3 * (size_t)g.nz * ncols * 2)) != QG_OK) { qg_destroy(c); return rc; }
   double2* tw = nullptr;

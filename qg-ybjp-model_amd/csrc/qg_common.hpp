@@ -55,7 +55,8 @@ Geo make_geo(int n1, int n2, int n3, int nm = 1);
 
 // flag bits, same values as include/qgybj.h
 enum : int {
-  F_FIXED_FLOW = 1, F_NO_FEEDBACK = 2, F_PASSIVE = 4, F_LINEAR = 8, F_NO_DISPERSION = 16, F_INVISCID = 32
+  F_FIXED_FLOW = 1, F_NO_FEEDBACK = 2, F_PASSIVE = 4, F_LINEAR = 8, F_NO_DISPERSION = 16, F_INVISCID = 32,
+  F_YBJ_REGULAR = 64, F_ALL = 127
 };
 
 // Kernel variant choice per hot kernel (ps_inv, row_nl, col_upd, row_qw,
@@ -81,6 +82,9 @@ struct Dev {
   // work: q* (and temporaries), dq, intermediates I0..I5 (pitch sp, n2 rows/level)
   double2 *qs, *dq;
   double2* I[6];
+  // regular YBJ only: the update's combined B tendencies nBR - rBI/2 and
+  // nBI + rBR/2 (box), summed over z for sigma; null otherwise
+  double2 *tR, *tI;
   double* scratch;  // Thomas elimination diagonals, 3 * n3 * ncols * 2
   // tables
   const double2* tw;       // TWN twiddles
@@ -127,6 +131,8 @@ hipError_t launch_clear_outside(const Dev& d, hipStream_t st, double2* f, int al
 hipError_t launch_init_q(const Dev& d, hipStream_t st, const double2* psi, double2* q);
 hipError_t launch_cz(const Dev& d, hipStream_t st, const double2* A, double2* C);
 hipError_t launch_diag(const Dev& d, hipStream_t st, int* nblocks);
+// regular YBJ: sumB, sigma and compute_A in one pass per (kx, ky) column
+hipError_t launch_ybj_regular(const Dev& d, hipStream_t st);
 // levels (1-based, as the reference's height array) of one spectra launch
 struct SpecLevels {
   static constexpr int MAXL = 32;

@@ -460,6 +460,86 @@ __global__ __launch_bounds__(NT) void k_diag(Dev d) {
 }
 
 // ---------------------------------------------------------------------------
+// Regular YBJ operator (ybj_plus = 0, main_waqg.f90:449-461), one thread per
+// (kx, ky) column of a member, in z order as the reference sums:
+//   sumB (derivatives.f90:1072-1118): B -= mean_z B
+//   compute_sigma (:1037-1067): sigma = Bu Ro sum_z (rBR + 2 nBI, rBI - 2 nBR)/kh2
+//     = Bu Ro sum_z (2 tI, -2 tR)/kh2 with the update's combined tendencies
+//     tR = nBR - rBI/2, tI = nBI + rBR/2
+//   compute_A (:1120-1231): A~_0 = 0, A~_k = A~_{k-1} + (sum_{j<k} B_j) N^2_u(k-1) dz^2,
+//     A = A~ + (sigma - sum_k A~_k)/n3
+// at the L = 1 modes with kh2 > 0; A = 0 at the other box modes.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void k_ybj_regular(Dev d) {
+  const Geo& g = d.g;
+  const int e = blockIdx.x * NT + threadIdx.x;
+  if (e >= g.KX * g.nbox) return;
+  const int kx = e % g.KX, iky = box_row(e / g.KX, g);
+  const size_t plane = (size_t)g.ikty * g.sp;
+  const size_t c0 = (size_t)blockIdx.y * g.n3 * plane + (size_t)iky * g.sp + kx;
+  const int ky = ky_of(iky, g.n2);
+  const int kh2i = kx * kx + ky * ky;
+  const int n3 = g.n3;
+  if (!lmask(kx, iky, g) || kh2i == 0) {
+    for (int z = 0; z < n3; ++z) {
+      d.AR[c0 + z * plane] = zc();
+      d.AI[c0 + z * plane] = zc();
+    }
+    return;
+  }
+  const double kh2 = (double)kh2i;
+  // pass 1: vertical sums of B (sumB) and of the tendencies (sigma)
+  double2 sR = zc(), sI = zc(), g1 = zc(), g2 = zc();
+#pragma unroll 8
+  for (int z = 0; z < n3; ++z) {
+    const size_t i = c0 + z * plane;
+    const double2 br = d.BR[i], bi = d.BI[i], tr = d.tR[i], ti = d.tI[i];
+    sR = make_double2(sR.x + br.x, sR.y + br.y);
+    sI = make_double2(sI.x + bi.x, sI.y + bi.y);
+    g1 = make_double2(g1.x + (2.0 * ti.x) / kh2, g1.y + (2.0 * ti.y) / kh2);
+    g2 = make_double2(g2.x + (-2.0 * tr.x) / kh2, g2.y + (-2.0 * tr.y) / kh2);
+  }
+  const double bro = d.Bu * d.Ro, dz2 = d.dz * d.dz;
+  const double2 mR = make_double2(sR.x / n3, sR.y / n3), mI = make_double2(sI.x / n3, sI.y / n3);
+  const double2 sgR = make_double2(g1.x * bro, g1.y * bro), sgI = make_double2(g2.x * bro, g2.y * bro);
+  // pass 2: the vertical sum of A~ from the mean-free B; pass 3 forms A~
+  // again, writes A = A~ + (sigma - sum A~)/n3 and the mean-free B
+  double2 cR = zc(), cI = zc();
+#pragma unroll 1
+  for (int pass = 0; pass < 2; ++pass) {
+    double2 aR = zc(), aI = zc(), sbR = zc(), sbI = zc(), saR = zc(), saI = zc();
+#pragma unroll 8
+    for (int z = 0; z < n3; ++z) {
+      const size_t i = c0 + z * plane;
+      const double2 br0 = d.BR[i], bi0 = d.BI[i];
+      const double2 br = make_double2(br0.x - mR.x, br0.y - mR.y), bi = make_double2(bi0.x - mI.x, bi0.y - mI.y);
+      if (z > 0) {
+        const double n2 = d.N2u[z - 1];
+        aR = make_double2(aR.x + sbR.x * n2 * dz2, aR.y + sbR.y * n2 * dz2);
+        aI = make_double2(aI.x + sbI.x * n2 * dz2, aI.y + sbI.y * n2 * dz2);
+        saR = make_double2(saR.x + aR.x, saR.y + aR.y);
+        saI = make_double2(saI.x + aI.x, saI.y + aI.y);
+        sbR = make_double2(sbR.x + br.x, sbR.y + br.y);
+        sbI = make_double2(sbI.x + bi.x, sbI.y + bi.y);
+      } else {
+        sbR = br;
+        sbI = bi;
+      }
+      if (pass == 1) {
+        d.AR[i] = make_double2(aR.x + cR.x, aR.y + cR.y);
+        d.AI[i] = make_double2(aI.x + cI.x, aI.y + cI.y);
+        d.BR[i] = br;
+        d.BI[i] = bi;
+      }
+    }
+    if (pass == 0) {
+      cR = make_double2((sgR.x - saR.x) / n3, (sgR.y - saR.y) / n3);
+      cI = make_double2((sgI.x - saI.x) / n3, (sgI.y - saI.y) / n3);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // hspec_waves (diagnostics.f90:547-638): horizontal spectra of the waves at
 // one level.  The L = 1 modes fall into shells mode = floor(|k| + 1/2); per
 // shell
@@ -738,6 +818,13 @@ hipError_t launch_diag(const Dev& d, hipStream_t st, int* nblocks) {
   const int nb = diag_blocks(d.g);
   *nblocks = nb;
   hipLaunchKernelGGL(k_diag, dim3(nb, d.g.nm), dim3(NT), 0, st, d);
+  return hipGetLastError();
+}
+
+hipError_t launch_ybj_regular(const Dev& d, hipStream_t st) {
+  if (!d.tR || !d.tI) return hipErrorInvalidValue;
+  const int nb = (d.g.KX * d.g.nbox + NT - 1) / NT;
+  hipLaunchKernelGGL(k_ybj_regular, dim3(nb, d.g.nm), dim3(NT), 0, st, d);
   return hipGetLastError();
 }
 

@@ -286,6 +286,8 @@ __global__ __launch_bounds__((SH<N, E>::COLT)) void k_col_upd(Dev d, int euler, 
       }
       X[idx] = xnew;
       if (!euler) Xo[idx] = xonew;
+      // the regular YBJ operator sums these tendencies over z (compute_sigma)
+      if (f > 0 && d.tR) (f == 1 ? d.tR : d.tI)[idx] = tt;
       bad |= !(isfinite(xnew.x) && isfinite(xnew.y));
       v[0][m] = xnew;
       v[1][m] = mul_ik(xnew, ky);

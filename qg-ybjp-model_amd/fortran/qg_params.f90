@@ -22,6 +22,7 @@ module qg_run_params
   double precision :: coeff1 = 0.d0, coeff2 = 10.d0, coeff1w = 0.d0, coeff2w = 10.d0, coeffz = 0.d0
   integer :: ilap1 = 4, ilap2 = 4, ilap1w = 4, ilap2w = 4
   integer :: fixed_flow = 0, no_feedback = 0, passive_scalar = 0, linear = 0, no_dispersion = 0, inviscid = 0
+  integer :: ybj_plus = 1              ! 0: the regular YBJ operator (sumB, compute_sigma, compute_A)
   integer :: stratification = constant_N
   double precision :: n2_ratio = 100.d0
   integer :: itermax = 10              ! steps to run (the first one is the Euler step)
@@ -54,7 +55,7 @@ module qg_run_params
 
   namelist /qg/ n1, n3, delt, gamma, Ro, Bu, W2F, coeff1, coeff2, coeff1w, coeff2w, coeffz, &
                 ilap1, ilap2, ilap1w, ilap2w, fixed_flow, no_feedback, passive_scalar, linear, &
-                no_dispersion, inviscid, stratification, n2_ratio, itermax, diag_every, out_every, &
+                no_dispersion, inviscid, ybj_plus, stratification, n2_ratio, itermax, diag_every, out_every, &
                 ic_prefix, out_prefix, grad_steps, grad_every, grad_zdeep, init_ncf_psi, init_ncf_la, &
                 init_ncf_psi_filename, init_ncf_la_filename, U_scale, Uw_scale, L_scale, N0, hspec_every
 
@@ -81,6 +82,7 @@ contains
     if (linear == 1) flag_bits = ior(flag_bits, 8)
     if (no_dispersion == 1) flag_bits = ior(flag_bits, 16)
     if (inviscid == 1) flag_bits = ior(flag_bits, 32)
+    if (ybj_plus == 0) flag_bits = ior(flag_bits, 64)
   end function flag_bits
 
   ! kxa, kya of init.f90:32-41 (the Nyquist row is relabelled 0)

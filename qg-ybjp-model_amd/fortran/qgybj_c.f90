@@ -13,7 +13,8 @@ module qgybj_c
   integer(c_int), parameter :: QG_OK = 0, QG_ERR_ARG = 1, QG_ERR_HIP = 2, QG_ERR_NOMEM = 3, &
                                QG_ERR_NONFINITE = 4, QG_ERR_STATE = 5
   integer(c_int32_t), parameter :: QG_FIXED_FLOW = 1, QG_NO_FEEDBACK = 2, QG_PASSIVE_SCALAR = 4, &
-                                   QG_LINEAR = 8, QG_NO_DISPERSION = 16, QG_INVISCID = 32
+                                   QG_LINEAR = 8, QG_NO_DISPERSION = 16, QG_INVISCID = 32, &
+                                   QG_YBJ_REGULAR = 64
 
   ! struct qg_params
   type, bind(C) :: qg_params_t

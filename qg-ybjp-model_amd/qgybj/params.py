@@ -21,6 +21,7 @@ PASSIVE_SCALAR = 4
 LINEAR = 8
 NO_DISPERSION = 16
 INVISCID = 32
+YBJ_REGULAR = 64  # ybj_plus = 0: the regular YBJ operator instead of YBJ+
 
 
 def _f32(x) -> float:

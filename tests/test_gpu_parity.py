@@ -117,6 +117,39 @@ def test_flag_variants(flags):
     m.close()
 
 
+@pytest.mark.parametrize("n1,n3,flags,strat", [
+    (32, 16, 0, "constant"),
+    (64, 8, qgybj.params.NO_FEEDBACK, "exponential"),
+    (32, 8, qgybj.params.LINEAR, "exponential"),
+    (32, 8, qgybj.params.NO_DISPERSION, "constant"),
+    (32, 8, qgybj.params.PASSIVE_SCALAR, "constant"),
+])
+def test_regular_ybj_operator(n1, n3, flags, strat):
+    """ybj_plus = 0 (QG_YBJ_REGULAR): sumB, compute_sigma and compute_A
+    (derivatives.f90:1037-1231) in place of the YBJ+ inversion, every state
+    field checked after every step, including the Euler step and the steps
+    whose B planes are re-formed from the mean-free B.  The regular operator
+    is only conditionally stable (dispersion frequency kh^2 N^2 / (2 Bu Ro m^2)
+    for vertical wavenumber m: the leapfrog needs a far smaller dt than
+    YBJ+, the oracle itself overflows at dt = 2e-3 on these grids), so the
+    runs use dt = 2e-5."""
+    P = make_params(n1, n3, dt=2e-5, W2F=0.05, stratification=strat, flags=flags | qgybj.params.YBJ_REGULAR)
+    m, S, g, OP, worst = _run_pair(P, 10, check_every=True)
+    print("worst rel L2 after 6 steps:", worst)
+    if not flags & qgybj.params.PASSIVE_SCALAR:
+        # B stays mean-free in z at every L = 1 mode with kh2 > 0
+        sel = g.Lb & (g.kh2 > 0)
+        st = m.get_state()
+        for f in ("BRk", "BIk"):
+            assert np.max(np.abs(st[f].sum(axis=0)[sel])) < 1e-12 * np.abs(st[f]).max()
+    m.close()
+
+
+def test_unknown_flag_rejected():
+    with pytest.raises(qgybj.QGError):
+        qgybj.Member(make_params(16, 4, flags=128))
+
+
 def test_vertical_diffusion_nuz():
     """nuz != 0 (coeffz > 0) exercises dissipation_q_nv including the
     Euler-step read of the old q."""
