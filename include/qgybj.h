@@ -140,6 +140,17 @@ int qg_diag(qg_ctx* ctx, double* out);
  * nondimensional, as the reference writes them to h<i>w.dat. */
 int qg_hspec_waves(qg_ctx* ctx, const int32_t* levels, int32_t nlev, double* out);
 
+/* Omega equation (omega_eqn_rhs, derivatives.f90:1469-1586; omega_equation,
+ * elliptic.f90:463-567): the QG vertical velocity w from the current psi, as
+ * a host array shaped like a state field ((iktx, ikty, n3) per member).
+ * Level k holds w at the unstaggered level z_u(k) = k dz: the interior
+ * interfaces k = 1 .. n3-1 solve
+ *   w_{k-1} - (2 + dz^2 kh2 N^2_u(k)/Bu) w_k + w_{k+1} = dz^2 rhs_k
+ * (w = 0 below the bottom and at the top level k = n3) with
+ * rhs = 2 J(psi_z, lap psi) formed by transforms; w = 0 at kh2 = 0 and
+ * outside the mask.  Nondimensional, as the reference's wak. */
+int qg_omega(qg_ctx* ctx, double* wk);
+
 /* init_q (init.f90:591-652) on host arrays: q = T(kh2) psi / dz^2 at L = 1. */
 int qg_init_q(qg_ctx* ctx, const double* psik, double* qk);
 

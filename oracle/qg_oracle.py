@@ -712,6 +712,60 @@ def diagnostics(grid: Grid, P: Params, S: State) -> np.ndarray:
     return np.array([KE, PE, WKE, WPE, WCE, ENS])
 
 
+def omega_eqn_rhs(grid: Grid, psik):
+    """derivatives.f90:1469-1586: 2 J(psi_z, lap psi) at the unstaggered
+    levels: psi_z = (psi_{k+1} - psi_k)/dz and the level average
+    (psi_{k+1} + psi_k)/2 at interface k (L = 1 modes), psi_z = 0 and psi
+    itself at the top level; bx, by = c2r(i k psi_z), xx, xy =
+    c2r(-i k kh2 psibar), rhs = r2c(2 (bx xy - by xx)) (unmasked)."""
+    L = grid.Lb
+    dz = grid.dz
+    kh2 = grid.kh2.astype(np.float64)
+    pz = np.zeros_like(psik)
+    pb = np.zeros_like(psik)
+    pz[:-1] = (psik[1:] - psik[:-1]) / dz
+    pb[:-1] = 0.5 * (psik[1:] + psik[:-1])
+    pb[-1] = psik[-1]
+    pz = np.where(L, pz, 0)
+    pb = np.where(L, pb, 0)
+    ikx, iky = 1j * grid.kx, 1j * grid.ky
+    bx = fft_c2r(ikx * pz)
+    by = fft_c2r(iky * pz)
+    xx = fft_c2r(-ikx * kh2 * pb)
+    xy = fft_c2r(-iky * kh2 * pb)
+    return fft_r2c(2.0 * (bx * xy - by * xx))
+
+
+def omega_equation(grid: Grid, a_ell_ut, rhs):
+    """elliptic.f90:463-567 with rho = 1: for kh2 != 0 and L = 1, the n3-1
+    interior interfaces solve w_{k-1} - (2 + dz^2 kh2 / a_ell_ut(k)) w_k +
+    w_{k+1} = dz^2 rhs_k (w = 0 beyond both ends; DGTSV, :503-525); the top
+    level and every other mode are 0."""
+    n3 = rhs.shape[0]
+    dz = grid.dz
+    sel = grid.Lb & (grid.kh2 > 0)
+    w = np.zeros_like(rhs)
+    m = n3 - 1
+    if m < 1:
+        return w
+    kh2 = grid.kh2[sel].astype(np.float64)
+    b = (dz * dz) * rhs[:m][:, sel]                      # (m, ncols)
+    d = -(2.0 + (dz * dz) * kh2[None, :] / np.asarray(a_ell_ut)[:m, None])
+    # Thomas sweep over all selected columns at once (no pivoting: |d| > 2)
+    c = np.zeros_like(d)
+    x = np.zeros_like(b)
+    c[0] = 1.0 / d[0]
+    x[0] = b[0] / d[0]
+    for k in range(1, m):
+        den = d[k] - c[k - 1]
+        c[k] = 1.0 / den
+        x[k] = (b[k] - x[k - 1]) / den
+    for k in range(m - 2, -1, -1):
+        x[k] = x[k] - c[k] * x[k + 1]
+    w[:m][:, sel] = x
+    return w
+
+
 def hspec_waves(grid: Grid, P: Params, S: State, height: int) -> np.ndarray:
     """hspec_waves (diagnostics.f90:547-638) at the 1-based level ``height``,
     summed in FP64 (the reference sums in REAL(4)): rows mode = 0 .. ktx-1

@@ -554,6 +554,33 @@ int qg_diag(qg_ctx* c, double* out) {
   return QG_OK;
 }
 
+int qg_omega(qg_ctx* c, double* wk) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!wk) return fail(c, QG_ERR_ARG, "null output");
+  Dev& d = c->d;
+  const Geo& g = d.g;
+  const size_t nsp = field_elems(g), nph = (size_t)g.n1 * g.n2 * g.nz;
+  // work fields of this diagnostic; the transforms' own scratch (q*) holds
+  // nothing between steps
+  void* blk = nullptr;
+  HIPCHK(c, hipMalloc(&blk, 5 * nsp * sizeof(double2) + 4 * nph * sizeof(double)));
+  double2* sp[4];
+  double* ph[4];
+  double2* base = static_cast<double2*>(blk);
+  for (int i = 0; i < 4; ++i) sp[i] = base + i * nsp;
+  double2* w = base + 4 * nsp;
+  double* pb = reinterpret_cast<double*>(base + 5 * nsp);
+  for (int i = 0; i < 4; ++i) ph[i] = pb + i * nph;
+  hipError_t e = hipMemsetAsync(w, 0, nsp * sizeof(double2), c->stream);
+  if (e == hipSuccess) e = qg::launch_omega(d, c->stream, sp, ph, w);
+  if (e == hipSuccess) e = d2h_field(g, wk, w, g.nz, c->stream, c->stage);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  hipFree(blk);
+  if (e != hipSuccess) return fail_hip(c, e, "qg_omega");
+  return QG_OK;
+}
+
 int qg_hspec_waves(qg_ctx* c, const int32_t* levels, int32_t nlev, double* out) {
   int rc = check_ctx(c);
   if (rc) return rc;

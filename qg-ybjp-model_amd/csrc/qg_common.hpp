@@ -131,6 +131,9 @@ hipError_t launch_clear_outside(const Dev& d, hipStream_t st, double2* f, int al
 hipError_t launch_init_q(const Dev& d, hipStream_t st, const double2* psi, double2* q);
 hipError_t launch_cz(const Dev& d, hipStream_t st, const double2* A, double2* C);
 hipError_t launch_diag(const Dev& d, hipStream_t st, int* nblocks);
+// omega equation: w (spectral, all levels) from psi, through four spectral
+// and four physical work fields (sp: ikty * sp * nz, ph: n1 * n2 * nz each)
+hipError_t launch_omega(const Dev& d, hipStream_t st, double2* const sp[4], double* const ph[4], double2* w);
 // regular YBJ: sumB, sigma and compute_A in one pass per (kx, ky) column
 hipError_t launch_ybj_regular(const Dev& d, hipStream_t st);
 // levels (1-based, as the reference's height array) of one spectra launch

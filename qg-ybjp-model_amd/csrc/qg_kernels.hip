@@ -540,6 +540,113 @@ __global__ __launch_bounds__(NT) void k_ybj_regular(Dev d) {
 }
 
 // ---------------------------------------------------------------------------
+// Omega equation (derivatives.f90:1469-1586, elliptic.f90:463-567), a
+// diagnostic: the right-hand side 2 J(psi_z, lap psi) at the unstaggered
+// levels through the raw transforms, then per (kx, ky) column the Dirichlet
+// problem w_{k-1} - (2 + dz^2 kh2 N^2_u(k)/Bu) w_k + w_{k+1} = dz^2 rhs_k on
+// the n3 - 1 interior interfaces (w = 0 at the top).
+// ---------------------------------------------------------------------------
+// bx, by = i k psi_z, xx, xy = -i k kh2 psibar at L = 1 (0 elsewhere), with
+// psi_z = (psi_{k+1} - psi_k)/dz and psibar = (psi_{k+1} + psi_k)/2 at
+// interface k, psi_z = 0 and psibar = psi at the top level of each member
+__global__ __launch_bounds__(NT) void k_omega_prep(Dev d, double2* __restrict__ bx, double2* __restrict__ by,
+                                                   double2* __restrict__ xx, double2* __restrict__ xy) {
+  const Geo& g = d.g;
+  const size_t plane = (size_t)g.ikty * g.sp;
+  const size_t e = (size_t)blockIdx.x * NT + threadIdx.x;
+  if (e >= plane * g.nz) return;
+  const int z = (int)(e / plane), r = (int)(e % plane);
+  const int iky = r / g.sp, kx = r % g.sp;
+  double2 vbx = zc(), vby = zc(), vxx = zc(), vxy = zc();
+  if (kx < g.iktx && lmask(kx, iky, g)) {
+    const bool top = (z % g.n3) == g.n3 - 1;
+    const double2 p0 = d.psi[e];
+    double2 pz = zc(), pb = p0;
+    if (!top) {
+      const double2 p1 = d.psi[e + plane];
+      pz = make_double2((p1.x - p0.x) / d.dz, (p1.y - p0.y) / d.dz);
+      pb = make_double2(0.5 * (p1.x + p0.x), 0.5 * (p1.y + p0.y));
+    }
+    const double k_x = (double)kx, k_y = (double)ky_of(iky, g.n2), kh2 = k_x * k_x + k_y * k_y;
+    vbx = make_double2(-k_x * pz.y, k_x * pz.x);
+    vby = make_double2(-k_y * pz.y, k_y * pz.x);
+    vxx = make_double2(k_x * kh2 * pb.y, -k_x * kh2 * pb.x);
+    vxy = make_double2(k_y * kh2 * pb.y, -k_y * kh2 * pb.x);
+  }
+  bx[e] = vbx;
+  by[e] = vby;
+  xx[e] = vxx;
+  xy[e] = vxy;
+}
+
+__global__ __launch_bounds__(NT) void k_omega_prod(const double* __restrict__ bx, const double* __restrict__ by,
+                                                   const double* __restrict__ xx, const double* __restrict__ xy,
+                                                   double* __restrict__ out, size_t n) {
+  const size_t e = (size_t)blockIdx.x * NT + threadIdx.x;
+  if (e < n) out[e] = 2.0 * (bx[e] * xy[e] - by[e] * xx[e]);
+}
+
+// Thomas sweep per box column of a member (no pivoting: the diagonal
+// dominates, |d| > 2 = |dl| + |du|); c' in the solver scratch
+__global__ __launch_bounds__(NT) void k_omega_solve(Dev d, const double2* __restrict__ rhs, double2* __restrict__ w) {
+  const Geo& g = d.g;
+  const int ncols = g.KX * g.nbox;
+  const int e = blockIdx.x * NT + threadIdx.x;
+  if (e >= ncols) return;
+  const int kx = e % g.KX, iky = box_row(e / g.KX, g);
+  const size_t plane = (size_t)g.ikty * g.sp;
+  const size_t c0 = (size_t)blockIdx.y * g.n3 * plane + (size_t)iky * g.sp + kx;
+  const int ky = ky_of(iky, g.n2);
+  const int kh2i = kx * kx + ky * ky;
+  const int m = g.n3 - 1;
+  if (!lmask(kx, iky, g) || kh2i == 0 || m < 1) {
+    for (int z = 0; z < g.n3; ++z) w[c0 + z * plane] = zc();
+    return;
+  }
+  double* cp = d.scratch + (size_t)blockIdx.y * g.n3 * ncols + e;
+  const double dz2 = d.dz * d.dz, kh2 = (double)kh2i;
+  double2 xp = zc();
+  double cprev = 0.0;
+  for (int k = 0; k < m; ++k) {
+    const double dk = -(2.0 + dz2 * kh2 / d.a_ell[k]);
+    const double den = k == 0 ? dk : dk - cprev;
+    const double2 b = rhs[c0 + k * plane];
+    const double2 bk = make_double2(dz2 * b.x, dz2 * b.y);
+    xp = k == 0 ? make_double2(bk.x / den, bk.y / den) : make_double2((bk.x - xp.x) / den, (bk.y - xp.y) / den);
+    cprev = 1.0 / den;
+    cp[(size_t)k * ncols] = cprev;
+    w[c0 + k * plane] = xp;
+  }
+  w[c0 + (size_t)m * plane] = zc();
+  double2 nx = xp;
+  for (int k = m - 2; k >= 0; --k) {
+    const double2 x = w[c0 + k * plane];
+    const double c = cp[(size_t)k * ncols];
+    nx = make_double2(x.x - c * nx.x, x.y - c * nx.y);
+    w[c0 + k * plane] = nx;
+  }
+}
+
+hipError_t launch_omega(const Dev& d, hipStream_t st, double2* const sp[4], double* const ph[4], double2* w) {
+  const Geo& g = d.g;
+  const size_t nsp = (size_t)g.ikty * g.sp * g.nz;
+  hipLaunchKernelGGL(k_omega_prep, dim3((unsigned)((nsp + NT - 1) / NT)), dim3(NT), 0, st, d, sp[0], sp[1], sp[2],
+                     sp[3]);
+  hipError_t e = hipGetLastError();
+  for (int i = 0; i < 4 && e == hipSuccess; ++i) e = launch_c2r(d, st, sp[i], ph[i], g.n1, g.nz);
+  if (e != hipSuccess) return e;
+  const size_t nph = (size_t)g.n1 * g.n2 * g.nz;
+  // 2 (bx xy - by xx), in the buffer of bx
+  hipLaunchKernelGGL(k_omega_prod, dim3((unsigned)((nph + NT - 1) / NT)), dim3(NT), 0, st, ph[0], ph[1], ph[2], ph[3],
+                     ph[0], nph);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = launch_r2c(d, st, ph[0], g.n1, sp[0], g.nz);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_omega_solve, dim3((g.KX * g.nbox + NT - 1) / NT, g.nm), dim3(NT), 0, st, d, sp[0], w);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // hspec_waves (diagnostics.f90:547-638): horizontal spectra of the waves at
 // one level.  The L = 1 modes fall into shells mode = floor(|k| + 1/2); per
 // shell

@@ -86,6 +86,12 @@ module qgybj_c
       real(c_double), intent(out) :: out(*)
     end function
 
+    ! QG vertical velocity w of the current psi (omega equation), shaped like a state field
+    integer(c_int) function qg_omega(ctx, wk) bind(C, name="qg_omega")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, wk
+    end function
+
     integer(c_int) function qg_fft_c2r(ctx, k, r, ldx, nz) bind(C, name="qg_fft_c2r")
       import :: c_int, c_ptr, c_int32_t
       type(c_ptr), value :: ctx, k, r

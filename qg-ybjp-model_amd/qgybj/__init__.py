@@ -28,7 +28,7 @@ KERNEL_NAMES = ["col_inv", "row_nl", "col_upd", "row_qw", "col_qw", "solve", "so
 # exported symbols declared in include/qgybj.h
 SYMBOLS = [
     "qg_create", "qg_create_batch", "qg_members", "qg_destroy", "qg_last_error", "qg_set_state", "qg_set_state_physical", "qg_get_state",
-    "qg_step", "qg_iter", "qg_diag", "qg_hspec_waves", "qg_init_q", "qg_fft_r2c", "qg_fft_c2r", "qg_device_field", "qg_device_pitch", "qg_stream",
+    "qg_step", "qg_iter", "qg_diag", "qg_hspec_waves", "qg_omega", "qg_init_q", "qg_fft_r2c", "qg_fft_c2r", "qg_device_field", "qg_device_pitch", "qg_stream",
     "qg_sync", "qg_timing", "qg_timing_sample", "qg_kernel_times", "qg_kernel_name", "qg_set_variants", "qg_get_variants",
     "qg_ensemble_create", "qg_ensemble_destroy", "qg_ensemble_error", "qg_ensemble_set_graph_steps",
     "qg_ensemble_step", "qg_tune", "qg_adj_create", "qg_adj_destroy", "qg_adj_error", "qg_adj_set_fused", "qg_adj_objective",
@@ -78,6 +78,7 @@ def load(path: str = LIB_PATH):
     lib.qg_iter.argtypes = [vp, C.POINTER(C.c_int32)]
     lib.qg_diag.argtypes = [vp, dp]
     lib.qg_hspec_waves.argtypes = [vp, C.POINTER(C.c_int32), C.c_int32, dp]
+    lib.qg_omega.argtypes = [vp, dp]
     lib.qg_init_q.argtypes = [vp, dp, dp]
     lib.qg_fft_r2c.argtypes = [vp, dp, C.c_int32, dp, C.c_int32]
     lib.qg_fft_c2r.argtypes = [vp, dp, dp, C.c_int32, C.c_int32]
@@ -231,6 +232,14 @@ class Member:
         self._chk(self.lib.qg_hspec_waves(self._h, lv.ctypes.data_as(C.POINTER(C.c_int32)), len(lv),
                                           _dp(out)), "qg_hspec_waves")
         return out[0] if self.members == 1 else out
+
+    def omega(self) -> np.ndarray:
+        """QG vertical velocity w (omega equation, elliptic.f90:463-567) of the
+        current state, shaped like a state field (w at z_u(k) = k dz, 0 at the
+        top level)."""
+        w = np.empty(self.shape, np.complex128)
+        self._chk(self.lib.qg_omega(self._h, _dp(w)), "qg_omega")
+        return w
 
     def init_q(self, psik):
         psik = self._c(psik)

@@ -1,9 +1,14 @@
-"""Device wave spectra (qg_hspec_waves, diagnostics.f90:547-638) through the
-C ABI against the oracle's restatement, at the reference's spectrum heights
-(parameters_test1.f90:188): shell counts exactly, the kinetic and potential
-parts to 1e-12; deterministic (bitwise equal on repeat); batch members equal
-single members bit for bit; more levels than one launch takes; bad levels
-rejected."""
+"""Device diagnostics through the C ABI against the oracle's restatements:
+
+* wave spectra (qg_hspec_waves, diagnostics.f90:547-638) at the reference's
+  spectrum heights (parameters_test1.f90:188): shell counts exactly, the
+  kinetic and potential parts to 1e-12; deterministic (bitwise equal on
+  repeat); batch members equal single members bit for bit; more levels than
+  one launch takes; bad levels rejected;
+* the omega equation (qg_omega, derivatives.f90:1469-1586 and
+  elliptic.f90:463-567) on a running state to 1e-11; the run continues
+  bit-identically; batch members equal single members.
+"""
 import numpy as np
 import pytest
 
@@ -65,5 +70,48 @@ def test_hspec_waves_batch_and_many_levels():
         one.set_state_physical(*ic)
         one.step(5)
         assert np.array_equal(one.hspec_waves(levels), gb[j])
+        one.close()
+    b.close()
+
+
+@pytest.mark.parametrize("n1,n3,strat", [(64, 16, "exponential"), (32, 8, "constant")])
+def test_omega_matches_oracle_and_leaves_the_run_unchanged(n1, n3, strat):
+    """qg_omega (omega equation, elliptic.f90:463-567) against the oracle on a
+    running state; a twin member that never calls it stays bit-identical."""
+    P = make_params(n1, n3, dt=2e-3, W2F=0.05, stratification=strat)
+    ics = physical_ics(P, k0=3, seed=6)
+    m, twin = qgybj.Member(P), qgybj.Member(P)
+    for x in (m, twin):
+        x.set_state_physical(*ics)
+        x.step(4)
+    got = m.omega()
+    g = O.Grid(n1, n1, n3)
+    want = O.omega_equation(g, P.Bu / P.N2u, O.omega_eqn_rhs(g, m.get_state()["psik"]))
+    assert np.linalg.norm(want) > 0
+    assert np.linalg.norm(got - want) <= 1e-11 * np.linalg.norm(want), np.linalg.norm(got - want) / np.linalg.norm(want)
+    assert np.all(got[-1] == 0)
+    sel = g.Lb & (g.kh2 > 0)
+    assert np.all(got[:, ~sel] == 0)
+    m.step(3)
+    twin.step(3)
+    a, b = m.get_state(), twin.get_state()
+    for f in a:
+        assert np.array_equal(a[f], b[f]), f
+    m.close()
+    twin.close()
+
+
+def test_omega_batch_members():
+    P = make_params(32, 8, dt=2e-3, W2F=0.05)
+    ics = [physical_ics(P, k0=3, seed=s) for s in (9, 10)]
+    b = qgybj.Member(P, members=2)
+    b.set_state_physical(*[np.stack([x[i] for x in ics]) for i in range(3)])
+    b.step(3)
+    wb = b.omega()
+    for j, ic in enumerate(ics):
+        one = qgybj.Member(P)
+        one.set_state_physical(*ic)
+        one.step(3)
+        assert np.array_equal(one.omega(), wb[j])
         one.close()
     b.close()
