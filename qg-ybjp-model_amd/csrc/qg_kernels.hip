@@ -460,8 +460,8 @@ __global__ __launch_bounds__(NT) void k_diag(Dev d) {
 }
 
 // ---------------------------------------------------------------------------
-// Regular YBJ operator (ybj_plus = 0, main_waqg.f90:449-461), one thread per
-// (kx, ky) column of a member, in z order as the reference sums:
+// Regular YBJ operator (ybj_plus = 0, main_waqg.f90:449-461), per (kx, ky)
+// column of a member:
 //   sumB (derivatives.f90:1072-1118): B -= mean_z B
 //   compute_sigma (:1037-1067): sigma = Bu Ro sum_z (rBR + 2 nBI, rBI - 2 nBR)/kh2
 //     = Bu Ro sum_z (2 tI, -2 tR)/kh2 with the update's combined tendencies
@@ -469,74 +469,141 @@ __global__ __launch_bounds__(NT) void k_diag(Dev d) {
 //   compute_A (:1120-1231): A~_0 = 0, A~_k = A~_{k-1} + (sum_{j<k} B_j) N^2_u(k-1) dz^2,
 //     A = A~ + (sigma - sum_k A~_k)/n3
 // at the L = 1 modes with kh2 > 0; A = 0 at the other box modes.
+// A workgroup takes YC columns and splits each column's levels into YZ
+// chunks, one thread per (column, chunk).  With the chunk's own running sum
+// lb of B and the weights w_k = N^2_u(k-1) dz^2 (w_0 = 0),
+//   A~_k = A~_in + sb_in W(k) + V(k),  W(k) = sum_{s<=j<=k} w_j,
+//   V(k) = sum_{s<=j<=k} w_j lb(j-1),
+// where A~_in, sb_in (the A~ and running B sum entering the chunk at level s)
+// come from combining the chunks' totals in order in LDS.  The sums are
+// associated differently from a single sweep, identically on every run.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(NT) void k_ybj_regular(Dev d) {
+constexpr int YC = 64, YZ = 4;
+
+__global__ __launch_bounds__(YC* YZ) void k_ybj_regular(Dev d) {
   const Geo& g = d.g;
-  const int e = blockIdx.x * NT + threadIdx.x;
-  if (e >= g.KX * g.nbox) return;
-  const int kx = e % g.KX, iky = box_row(e / g.KX, g);
+  const int c = threadIdx.x % YC, q = threadIdx.x / YC;
+  const int ncols = g.KX * g.nbox;
+  const int e = blockIdx.x * YC + c;
+  const bool colok = e < ncols;
+  const int ec = colok ? e : 0;
+  const int kx = ec % g.KX, iky = box_row(ec / g.KX, g);
   const size_t plane = (size_t)g.ikty * g.sp;
   const size_t c0 = (size_t)blockIdx.y * g.n3 * plane + (size_t)iky * g.sp + kx;
   const int ky = ky_of(iky, g.n2);
   const int kh2i = kx * kx + ky * ky;
-  const int n3 = g.n3;
-  if (!lmask(kx, iky, g) || kh2i == 0) {
-    for (int z = 0; z < n3; ++z) {
+  const int n3 = g.n3, len = (n3 + YZ - 1) / YZ;
+  const int z0 = q * len < n3 ? q * len : n3, z1 = z0 + len < n3 ? z0 + len : n3;
+  const bool act = colok && lmask(kx, iky, g) && kh2i != 0;
+  __shared__ double2 sh[8][YZ][YC];
+  __shared__ double shr[2][YZ][YC];
+  // pass 1: chunk sums of B and of the sigma terms
+  double2 sR = zc(), sI = zc(), g1 = zc(), g2 = zc();
+  const double kh2 = (double)(kh2i > 0 ? kh2i : 1);
+  if (act) {
+    for (int z = z0; z < z1; ++z) {
+      const size_t i = c0 + z * plane;
+      const double2 br = d.BR[i], bi = d.BI[i], tr = d.tR[i], ti = d.tI[i];
+      sR = make_double2(sR.x + br.x, sR.y + br.y);
+      sI = make_double2(sI.x + bi.x, sI.y + bi.y);
+      g1 = make_double2(g1.x + (2.0 * ti.x) / kh2, g1.y + (2.0 * ti.y) / kh2);
+      g2 = make_double2(g2.x + (-2.0 * tr.x) / kh2, g2.y + (-2.0 * tr.y) / kh2);
+    }
+  }
+  sh[0][q][c] = sR;
+  sh[1][q][c] = sI;
+  sh[2][q][c] = g1;
+  sh[3][q][c] = g2;
+  __syncthreads();
+  sR = zc(); sI = zc(); g1 = zc(); g2 = zc();
+  for (int p = 0; p < YZ; ++p) {
+    sR = make_double2(sR.x + sh[0][p][c].x, sR.y + sh[0][p][c].y);
+    sI = make_double2(sI.x + sh[1][p][c].x, sI.y + sh[1][p][c].y);
+    g1 = make_double2(g1.x + sh[2][p][c].x, g1.y + sh[2][p][c].y);
+    g2 = make_double2(g2.x + sh[3][p][c].x, g2.y + sh[3][p][c].y);
+  }
+  __syncthreads();
+  const double bro = d.Bu * d.Ro, dz2 = d.dz * d.dz;
+  const double2 mR = make_double2(sR.x / n3, sR.y / n3), mI = make_double2(sI.x / n3, sI.y / n3);
+  const double2 sgR = make_double2(g1.x * bro, g1.y * bro), sgI = make_double2(g2.x * bro, g2.y * bro);
+  // pass 2: chunk totals L (sum b), W, V and the sums of W and V over the chunk
+  double2 lbR = zc(), lbI = zc(), vR = zc(), vI = zc(), svR = zc(), svI = zc();
+  double W = 0.0, sW = 0.0;
+  if (act) {
+    for (int z = z0; z < z1; ++z) {
+      const size_t i = c0 + z * plane;
+      const double2 br0 = d.BR[i], bi0 = d.BI[i];
+      const double wz = z > 0 ? d.N2u[z - 1] * dz2 : 0.0;
+      W += wz;
+      vR = make_double2(vR.x + wz * lbR.x, vR.y + wz * lbR.y);
+      vI = make_double2(vI.x + wz * lbI.x, vI.y + wz * lbI.y);
+      sW += W;
+      svR = make_double2(svR.x + vR.x, svR.y + vR.y);
+      svI = make_double2(svI.x + vI.x, svI.y + vI.y);
+      lbR = make_double2(lbR.x + (br0.x - mR.x), lbR.y + (br0.y - mR.y));
+      lbI = make_double2(lbI.x + (bi0.x - mI.x), lbI.y + (bi0.y - mI.y));
+    }
+  }
+  sh[0][q][c] = lbR;
+  sh[1][q][c] = lbI;
+  sh[2][q][c] = vR;
+  sh[3][q][c] = vI;
+  sh[4][q][c] = svR;
+  sh[5][q][c] = svI;
+  shr[0][q][c] = W;
+  shr[1][q][c] = sW;
+  __syncthreads();
+  // in-order combine: A~ and the running B sum entering this chunk, sum of A~
+  double2 aR = zc(), aI = zc(), sbR = zc(), sbI = zc(), saR = zc(), saI = zc();
+  double2 inAR = zc(), inAI = zc(), inSR = zc(), inSI = zc();
+  for (int p = 0; p < YZ; ++p) {
+    if (p == q) {
+      inAR = aR; inAI = aI; inSR = sbR; inSI = sbI;
+    }
+    const int n0 = p * len < n3 ? p * len : n3, n1 = n0 + len < n3 ? n0 + len : n3;
+    const double np = (double)(n1 - n0), Wp = shr[0][p][c], sWp = shr[1][p][c];
+    saR = make_double2(saR.x + (np * aR.x + sbR.x * sWp + sh[4][p][c].x), saR.y + (np * aR.y + sbR.y * sWp + sh[4][p][c].y));
+    saI = make_double2(saI.x + (np * aI.x + sbI.x * sWp + sh[5][p][c].x), saI.y + (np * aI.y + sbI.y * sWp + sh[5][p][c].y));
+    aR = make_double2(aR.x + sbR.x * Wp + sh[2][p][c].x, aR.y + sbR.y * Wp + sh[2][p][c].y);
+    aI = make_double2(aI.x + sbI.x * Wp + sh[3][p][c].x, aI.y + sbI.y * Wp + sh[3][p][c].y);
+    sbR = make_double2(sbR.x + sh[0][p][c].x, sbR.y + sh[0][p][c].y);
+    sbI = make_double2(sbI.x + sh[1][p][c].x, sbI.y + sh[1][p][c].y);
+  }
+  if (!colok) return;
+  if (!act) {
+    for (int z = z0; z < z1; ++z) {
       d.AR[c0 + z * plane] = zc();
       d.AI[c0 + z * plane] = zc();
     }
     return;
   }
-  const double kh2 = (double)kh2i;
-  // pass 1: vertical sums of B (sumB) and of the tendencies (sigma)
-  double2 sR = zc(), sI = zc(), g1 = zc(), g2 = zc();
-#pragma unroll 8
-  for (int z = 0; z < n3; ++z) {
+  const double2 cR = make_double2((sgR.x - saR.x) / n3, (sgR.y - saR.y) / n3);
+  const double2 cI = make_double2((sgI.x - saI.x) / n3, (sgI.y - saI.y) / n3);
+  // pass 3: A = A~ + (sigma - sum A~)/n3 and the mean-free B
+  lbR = zc(); lbI = zc(); vR = zc(); vI = zc();
+  W = 0.0;
+  for (int z = z0; z < z1; ++z) {
     const size_t i = c0 + z * plane;
-    const double2 br = d.BR[i], bi = d.BI[i], tr = d.tR[i], ti = d.tI[i];
-    sR = make_double2(sR.x + br.x, sR.y + br.y);
-    sI = make_double2(sI.x + bi.x, sI.y + bi.y);
-    g1 = make_double2(g1.x + (2.0 * ti.x) / kh2, g1.y + (2.0 * ti.y) / kh2);
-    g2 = make_double2(g2.x + (-2.0 * tr.x) / kh2, g2.y + (-2.0 * tr.y) / kh2);
+    const double2 br0 = d.BR[i], bi0 = d.BI[i];
+    const double2 br = make_double2(br0.x - mR.x, br0.y - mR.y), bi = make_double2(bi0.x - mI.x, bi0.y - mI.y);
+    const double wz = z > 0 ? d.N2u[z - 1] * dz2 : 0.0;
+    W += wz;
+    vR = make_double2(vR.x + wz * lbR.x, vR.y + wz * lbR.y);
+    vI = make_double2(vI.x + wz * lbI.x, vI.y + wz * lbI.y);
+    d.AR[i] = make_double2(inAR.x + inSR.x * W + vR.x + cR.x, inAR.y + inSR.y * W + vR.y + cR.y);
+    d.AI[i] = make_double2(inAI.x + inSI.x * W + vI.x + cI.x, inAI.y + inSI.y * W + vI.y + cI.y);
+    d.BR[i] = br;
+    d.BI[i] = bi;
+    lbR = make_double2(lbR.x + br.x, lbR.y + br.y);
+    lbI = make_double2(lbI.x + bi.x, lbI.y + bi.y);
   }
-  const double bro = d.Bu * d.Ro, dz2 = d.dz * d.dz;
-  const double2 mR = make_double2(sR.x / n3, sR.y / n3), mI = make_double2(sI.x / n3, sI.y / n3);
-  const double2 sgR = make_double2(g1.x * bro, g1.y * bro), sgI = make_double2(g2.x * bro, g2.y * bro);
-  // pass 2: the vertical sum of A~ from the mean-free B; pass 3 forms A~
-  // again, writes A = A~ + (sigma - sum A~)/n3 and the mean-free B
-  double2 cR = zc(), cI = zc();
-#pragma unroll 1
-  for (int pass = 0; pass < 2; ++pass) {
-    double2 aR = zc(), aI = zc(), sbR = zc(), sbI = zc(), saR = zc(), saI = zc();
-#pragma unroll 8
-    for (int z = 0; z < n3; ++z) {
-      const size_t i = c0 + z * plane;
-      const double2 br0 = d.BR[i], bi0 = d.BI[i];
-      const double2 br = make_double2(br0.x - mR.x, br0.y - mR.y), bi = make_double2(bi0.x - mI.x, bi0.y - mI.y);
-      if (z > 0) {
-        const double n2 = d.N2u[z - 1];
-        aR = make_double2(aR.x + sbR.x * n2 * dz2, aR.y + sbR.y * n2 * dz2);
-        aI = make_double2(aI.x + sbI.x * n2 * dz2, aI.y + sbI.y * n2 * dz2);
-        saR = make_double2(saR.x + aR.x, saR.y + aR.y);
-        saI = make_double2(saI.x + aI.x, saI.y + aI.y);
-        sbR = make_double2(sbR.x + br.x, sbR.y + br.y);
-        sbI = make_double2(sbI.x + bi.x, sbI.y + bi.y);
-      } else {
-        sbR = br;
-        sbI = bi;
-      }
-      if (pass == 1) {
-        d.AR[i] = make_double2(aR.x + cR.x, aR.y + cR.y);
-        d.AI[i] = make_double2(aI.x + cI.x, aI.y + cI.y);
-        d.BR[i] = br;
-        d.BI[i] = bi;
-      }
-    }
-    if (pass == 0) {
-      cR = make_double2((sgR.x - saR.x) / n3, (sgR.y - saR.y) / n3);
-      cI = make_double2((sgI.x - saI.x) / n3, (sgI.y - saI.y) / n3);
-    }
-  }
+}
+
+hipError_t launch_ybj_regular(const Dev& d, hipStream_t st) {
+  if (!d.tR || !d.tI) return hipErrorInvalidValue;
+  const int nb = (d.g.KX * d.g.nbox + YC - 1) / YC;
+  hipLaunchKernelGGL(k_ybj_regular, dim3(nb, d.g.nm), dim3(YC * YZ), 0, st, d);
+  return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
@@ -925,13 +992,6 @@ hipError_t launch_diag(const Dev& d, hipStream_t st, int* nblocks) {
   const int nb = diag_blocks(d.g);
   *nblocks = nb;
   hipLaunchKernelGGL(k_diag, dim3(nb, d.g.nm), dim3(NT), 0, st, d);
-  return hipGetLastError();
-}
-
-hipError_t launch_ybj_regular(const Dev& d, hipStream_t st) {
-  if (!d.tR || !d.tI) return hipErrorInvalidValue;
-  const int nb = (d.g.KX * d.g.nbox + NT - 1) / NT;
-  hipLaunchKernelGGL(k_ybj_regular, dim3(nb, d.g.nm), dim3(NT), 0, st, d);
   return hipGetLastError();
 }
 
