@@ -151,6 +151,39 @@ int qg_hspec_waves(qg_ctx* ctx, const int32_t* levels, int32_t nlev, double* out
  * outside the mask.  Nondimensional, as the reference's wak. */
 int qg_omega(qg_ctx* ctx, double* wk);
 
+/* Keep the wave tendencies of every step (on != 0) or stop keeping them.
+ * The reference keeps its tendency arrays nBRk, nBIk (convol_waqg,
+ * derivatives.f90:607-803) and rBRk, rBIk (refraction_waqg :810-900) from
+ * the last step, and its conversion diagnostics read them after the step
+ * (main_waqg.f90:506-507).  The fused step never forms them separately, so
+ * while this is on each step also writes F(zeta BR), F(zeta BI) planes and
+ * the combined tendencies and splits them afterwards (four more fields of
+ * device memory, allocated on the first call; the step costs more).  A new
+ * state (qg_set_state*, qg_tune) discards kept tendencies. */
+int qg_keep_tendencies(qg_ctx* ctx, int32_t on);
+
+/* The last step's nBRk, nBIk, rBRk, rBIk as host arrays shaped like state
+ * fields, masked by L; under QG_LINEAR nB = 0, under QG_PASSIVE rB = 0
+ * (main_waqg.f90:338-358).  QG_ERR_STATE unless a step has run with
+ * tendencies kept since the last state change.  NULL arrays are skipped. */
+int qg_get_tendencies(qg_ctx* ctx, double* nBRk, double* nBIk, double* rBRk, double* rBIk);
+
+/* Wave energy conversion terms from the kept tendencies of the last step and
+ * the current A, B (as the reference evaluates them after the step,
+ * main_waqg.f90:506-507), 5 values per member, member-major, in FP64 (the
+ * reference sums in REAL(4)) and without the dimensional factors:
+ * out[0], out[1]: we_conversion ca, cr (diagnostics.f90:967-1076), to be
+ *   multiplied by Uw_scale^2;
+ * out[2], out[3], out[4]: gamma_conversion ca, cr, cd (:1080-1256), to be
+ *   multiplied by Uw_scale^2 U_scale / L_scale.
+ * With D = lap A = -kh2 A and volume means <.> over the n1 n2 n3 points:
+ * ca = <DR nBR + DI nBI> / (2 Bu), cr = <DI rBR - DR rBI> / (4 Bu),
+ * gamma ca = -<DR nBR + DI nBI> / (2 Bu), gamma cr = -<DI rBR - DR rBI> / (4 Bu),
+ * cd = <DR dR + DI dI> / Bu with d = -(1/2)(nuh1w (kx^2p1 + ky^2p1) +
+ * nuh2w (kx^2p2 + ky^2p2)) B, the wave integrating-factor operator without
+ * delt.  QG_ERR_STATE as for qg_get_tendencies. */
+int qg_wave_conversion(qg_ctx* ctx, double* out);
+
 /* init_q (init.f90:591-652) on host arrays: q = T(kh2) psi / dz^2 at L = 1. */
 int qg_init_q(qg_ctx* ctx, const double* psik, double* qk);
 

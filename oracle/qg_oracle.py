@@ -23,7 +23,13 @@ need it (tests/test_oracle_*.py):
                        dispersion-only rotation rate from the analytic
                        vertical eigenvalue;
 * wave spectra      -- single-mode and linear-in-z closed forms, shell counts
-                       equal to the retained-mode counts.
+                       equal to the retained-mode counts;
+* regular YBJ       -- the discrete YBJ relation inverted by compute_A, zero
+                       vertical sums after sumB, sigma keeping sum_z B fixed;
+* omega equation    -- a two-mode closed-form right-hand side, manufactured
+                       solutions and LAPACK dgtsv;
+* conversion terms  -- single-mode closed forms by Parseval and the
+                       tendencies' reconstruction of the leapfrog update.
 
 Array convention: spectral fields are complex128 arrays of shape
 (nz, ikty, iktx) -- byte-for-byte the reference's Fortran ``(iktx,ikty,nz)``
@@ -521,6 +527,9 @@ class State:
     CRk: np.ndarray
     CIk: np.ndarray
     iter: int = 0
+    # (nBRk, nBIk, rBRk, rBIk) of the step that produced this state, as the
+    # reference's tendency arrays hold them after the step; None before
+    tend: tuple | None = None
 
     def copy(self) -> "State":
         return State(*[np.array(getattr(self, f.name), copy=True) if isinstance(getattr(self, f.name), np.ndarray)
@@ -612,6 +621,7 @@ def euler_step(grid: Grid, P: Params, S: State) -> State:
     S.BIk = np.where(L, BI1, 0).astype(np.complex128)
     _invert(grid, P, S, (nBRk, nBIk, rBRk, rBIk))
     S.iter = 1
+    S.tend = (nBRk, nBIk, rBRk, rBIk)
     return S
 
 
@@ -648,6 +658,7 @@ def leapfrog_step(grid: Grid, P: Params, S: State) -> State:
     S.BIk = BIt.astype(np.complex128)
     _invert(grid, P, S, (nBRk, nBIk, rBRk, rBIk))
     S.iter += 1
+    S.tend = (nBRk, nBIk, rBRk, rBIk)
     return S
 
 
@@ -799,6 +810,39 @@ def hspec_waves(grid: Grid, P: Params, S: State, height: int) -> np.ndarray:
         np.add.at(out[:, col], mode[L], val[L])
     out[:, :2] *= 0.5
     return out[:ktx]
+
+
+def wave_conversion(grid: Grid, P: Params, S: State) -> np.ndarray:
+    """[we ca, we cr, gamma ca, gamma cr, gamma cd]: we_conversion
+    (diagnostics.f90:967-1076) and gamma_conversion (:1080-1256) of the state
+    and the tendencies of the step that produced it, as physical-space sums in
+    FP64 (the reference accumulates in REAL(4)), without the dimensional
+    factors Uw_scale^2 (we) and Uw_scale^2 U_scale / L_scale (gamma):
+      nA = -kh2 A at every mode (:1000-1011 / :1125-1135), c2r of nA and of
+      nB, rB (:1014-1018, :1049-1050);
+      ca_p = sum nAR nBR + nAI nBI (:1025),  cr_p = sum 0.5 (-nAR rBI + nAI rBR)
+      (:1058), normalised by 2 n1 n2 n3 Bu (:1071-1072);
+      gamma: ca_p = -0.5 sum (nAR nBR + nAI nBI) (:1163),
+      cr_p = 0.25 sum (nAR rBI - nAI rBR) (:1198), cd_p = sum nAR tR + nAI tI with
+      t = -0.5 (nuh1w (kx^2p1 + ky^2p1) + nuh2w (kx^2p2 + ky^2p2)) B (:1210-1241),
+      normalised by n1 n2 n3 Bu (:1249-1251).
+    The dissipation operator is int_factor_w / delt (main_waqg.f90:372), the
+    same product up to the rounding of the multiplication by delt."""
+    nBR, nBI, rBR, rBI = S.tend
+    kh2 = grid.kh2.astype(np.float64)
+    nAR = fft_c2r(-kh2 * S.ARk)
+    nAI = fft_c2r(-kh2 * S.AIk)
+    ca = np.sum(nAR * fft_c2r(nBR) + nAI * fft_c2r(nBI))
+    cr = np.sum(0.5 * (-nAR * fft_c2r(rBI) + nAI * fft_c2r(rBR)))
+    coef = P.ifac_w / P.dt
+    tR = fft_c2r(-0.5 * coef * S.BRk)
+    tI = fft_c2r(-0.5 * coef * S.BIk)
+    g_ca = np.sum(-0.5 * (nAR * fft_c2r(nBR) + nAI * fft_c2r(nBI)))
+    g_cr = np.sum(0.25 * (nAR * fft_c2r(rBI) - nAI * fft_c2r(rBR)))
+    g_cd = np.sum(nAR * tR + nAI * tI)
+    nv = float(grid.n1 * grid.n2 * grid.n3)
+    return np.array([ca / (2.0 * nv * P.Bu), cr / (2.0 * nv * P.Bu),
+                     g_ca / (nv * P.Bu), g_cr / (nv * P.Bu), g_cd / (nv * P.Bu)])
 
 
 def rel_l2(a, b, grid: Grid | None = None) -> float:

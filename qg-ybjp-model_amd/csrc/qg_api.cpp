@@ -230,6 +230,11 @@ int step_once(qg_ctx* c) {
   }
   // after compute_qw has used B^{n+1} (main_waqg.f90:449-461)
   if (regular) KRUN(c, qg::K_SOLVE_A, qg::launch_ybj_regular(d, st));
+  // kept tendencies: after the regular operator has summed the combined ones
+  if (d.Z[0]) {
+    KRUN(c, qg::K_AUX, qg::launch_tend_split(d, st));
+    c->tend_valid = 1;
+  }
   if (c->clear_pending > 0) {
     if (c->evA_live) HIPCHK(c, hipStreamWaitEvent(st, c->evA, 0));
     // modes outside the dealiasing box are zero in the reference after the
@@ -376,22 +381,24 @@ int qg_create_batch(const qg_params* p, int32_t members, qg_ctx** out) {
   if ((rc = dalloc(c, &tw, (size_t)qg::TWN)) != QG_OK) { qg_destroy(c); return rc; }
   d.tw = tw;
   double* tab = nullptr;
-  if ((rc = dalloc(c, &tab, 4 * plane + 6 * (size_t)g.n3)) != QG_OK) { qg_destroy(c); return rc; }
+  if ((rc = dalloc(c, &tab, 5 * plane + 6 * (size_t)g.n3)) != QG_OK) { qg_destroy(c); return rc; }
   if ((rc = dalloc(c, &d.diag_part, (size_t)qg::diag_blocks(g) * g.nm * 6)) != QG_OK) { qg_destroy(c); return rc; }
   if ((rc = dalloc(c, &d.flag, 1)) != QG_OK) { qg_destroy(c); return rc; }
 
   // host tables
   std::vector<double2> htw;
   make_twiddles(htw);
-  std::vector<double> h(4 * plane + 6 * (size_t)g.n3);
+  std::vector<double> h(5 * plane + 6 * (size_t)g.n3);
   for (size_t i = 0; i < plane; ++i) {
     h[i] = std::exp(-p->ifac_q[i]);
     h[plane + i] = std::exp(-2.0 * p->ifac_q[i]);
     h[2 * plane + i] = std::exp(-p->ifac_w[i]);
     h[3 * plane + i] = std::exp(-2.0 * p->ifac_w[i]);
+    // the wave dissipation operator without delt (diagnostics.f90:1218-1219)
+    h[4 * plane + i] = p->dt != 0.0 ? p->ifac_w[i] / p->dt : 0.0;
   }
   const int n3 = g.n3;
-  double* hv = h.data() + 4 * plane;
+  double* hv = h.data() + 5 * plane;
   double* hdl = hv;
   double* hdu = hv + n3;
   double* hds = hv + 2 * n3;
@@ -421,7 +428,8 @@ int qg_create_batch(const qg_params* p, int32_t members, qg_ctx** out) {
   d.e2q = tab + plane;
   d.e1w = tab + 2 * plane;
   d.e2w = tab + 3 * plane;
-  const double* tv = tab + 4 * plane;
+  d.cw = tab + 4 * plane;
+  const double* tv = tab + 5 * plane;
   d.dl = tv;
   d.du = tv + n3;
   d.dsum = tv + 2 * n3;
@@ -457,6 +465,7 @@ int qg_set_state(qg_ctx* c, const double* psik, const double* qk, const double* 
   c->iter = iter;
   c->p_valid = 0;
   c->clear_pending = 2;
+  c->tend_valid = 0;
   return QG_OK;
 }
 
@@ -489,6 +498,7 @@ int qg_set_state_physical(qg_ctx* c, const double* psir, const double* BRr, cons
   c->iter = 0;
   c->p_valid = 0;
   c->clear_pending = 2;
+  c->tend_valid = 0;
   return QG_OK;
 }
 
@@ -551,6 +561,89 @@ int qg_diag(qg_ctx* c, double* out) {
       for (int b = 0; b < nb; ++b) s += part[((size_t)m * nb + b) * 6 + j];
       out[m * 6 + j] = s / c->d.g.n3;
     }
+  return QG_OK;
+}
+
+int qg_keep_tendencies(qg_ctx* c, int32_t on) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  Dev& d = c->d;
+  const bool regular = (d.flags & qg::F_YBJ_REGULAR) != 0;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream2));
+  if (on) {
+    const size_t S = field_elems(d.g);
+    for (int i = 0; i < 2; ++i) {
+      if (!c->kz[i]) {
+        if ((rc = dalloc(c, &c->kz[i], S)) != QG_OK) return fail(c, rc, "no memory for the kept tendencies");
+        // the columns kx >= KX of the planes are never written by the step
+        HIPCHK(c, hipMemset(c->kz[i], 0, S * sizeof(double2)));
+      }
+      if (!c->kt[i]) {
+        if (regular) {
+          c->kt[i] = i == 0 ? d.tR : d.tI;
+        } else {
+          if ((rc = dalloc(c, &c->kt[i], S)) != QG_OK) return fail(c, rc, "no memory for the kept tendencies");
+          HIPCHK(c, hipMemset(c->kt[i], 0, S * sizeof(double2)));
+        }
+      }
+    }
+    if (!c->keep_tend) c->tend_valid = 0;
+    d.Z[0] = c->kz[0];
+    d.Z[1] = c->kz[1];
+    d.tR = c->kt[0];
+    d.tI = c->kt[1];
+  } else {
+    d.Z[0] = d.Z[1] = nullptr;
+    if (!regular) d.tR = d.tI = nullptr;
+    c->tend_valid = 0;
+  }
+  c->keep_tend = on ? 1 : 0;
+  c->generation++;  // captured launch sequences change
+  return QG_OK;
+}
+
+int qg_get_tendencies(qg_ctx* c, double* nBRk, double* nBIk, double* rBRk, double* rBIk) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!c->keep_tend || !c->tend_valid) return fail(c, QG_ERR_STATE, "no tendencies kept since the last state change");
+  Dev& d = c->d;
+  double* dst[] = {nBRk, nBIk, rBRk, rBIk};
+  const double2* src[] = {d.tR, d.tI, d.Z[0], d.Z[1]};
+  HIPCHK(c, hipStreamWaitEvent(c->stream, c->evA, 0));
+  for (int i = 0; i < 4; ++i)
+    if (dst[i]) HIPCHK(c, d2h_field(d.g, dst[i], src[i], d.g.nz, c->stream, c->stage));
+  return QG_OK;
+}
+
+int qg_wave_conversion(qg_ctx* c, double* out) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!out) return fail(c, QG_ERR_ARG, "null output");
+  if (!c->keep_tend || !c->tend_valid) return fail(c, QG_ERR_STATE, "no tendencies kept since the last state change");
+  int nb = 0;
+  HIPCHK(c, hipStreamWaitEvent(c->stream, c->evA, 0));  // A^{n+1} from the side stream
+  HIPCHK(c, qg::launch_conv(c->d, c->stream, &nb));
+  const int nm = c->d.g.nm;
+  std::vector<double> part((size_t)nb * nm * 6);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(part.data(), c->d.diag_part, part.size() * sizeof(double), hipMemcpyDeviceToHost));
+  // volume means of the physical-space sums (sum_x = n1 n2 sum_k), then the
+  // normalisations of diagnostics.f90:1071-1072 and :1249-1251 without the
+  // dimensional scales Uw_scale^2 and Uw_scale^2 U_scale / L_scale
+  const double n3 = c->d.g.n3, Bu = c->d.Bu;
+  for (int m = 0; m < nm; ++m) {
+    double S[3] = {0.0, 0.0, 0.0};
+    for (int j = 0; j < 3; ++j)
+      for (int b = 0; b < nb; ++b) S[j] += part[((size_t)m * nb + b) * 6 + j];
+    for (int j = 0; j < 3; ++j) S[j] /= n3;
+    double* o = out + (size_t)m * 5;
+    o[0] = S[0] / (2.0 * Bu);          // we_conversion ca
+    o[1] = 0.5 * S[1] / (2.0 * Bu);    // we_conversion cr
+    o[2] = -0.5 * S[0] / Bu;           // gamma_conversion ca
+    o[3] = -0.25 * S[1] / Bu;          // gamma_conversion cr
+    o[4] = S[2] / Bu;                  // gamma_conversion cd
+  }
   return QG_OK;
 }
 

@@ -85,10 +85,15 @@ struct Dev {
   // regular YBJ only: the update's combined B tendencies nBR - rBI/2 and
   // nBI + rBR/2 (box), summed over z for sigma; null otherwise
   double2 *tR, *tI;
+  // kept B tendencies (qg_keep_tendencies), null otherwise: row_nl writes
+  // F(zeta BR), F(zeta BI) as (kx, y) planes into Z, which the tendency split
+  // turns into rBR, rBI in place; tR, tI then become nBR, nBI
+  double2* Z[2];
   double* scratch;  // Thomas elimination diagonals, 3 * n3 * ncols * 2
   // tables
   const double2* tw;       // TWN twiddles
   const double *e1q, *e2q, *e1w, *e2w;  // exp(-f), exp(-2f) per (iky, kx)
+  const double* cw;                     // wave dissipation coefficient fw / dt per (iky, kx)
   const double *dl, *du, *dsum;         // vertical operator, n3 each
   const double *a_ell, *N2u, *N2s;      // n3 each
   double* diag_part;                    // per-block partial sums
@@ -136,6 +141,12 @@ hipError_t launch_diag(const Dev& d, hipStream_t st, int* nblocks);
 hipError_t launch_omega(const Dev& d, hipStream_t st, double2* const sp[4], double* const ph[4], double2* w);
 // regular YBJ: sumB, sigma and compute_A in one pass per (kx, ky) column
 hipError_t launch_ybj_regular(const Dev& d, hipStream_t st);
+// kept tendencies: y transform of the Z planes and the split of the update's
+// combined tendencies into nBR, nBI (in tR, tI) and rBR, rBI (in Z), masked
+hipError_t launch_tend_split(const Dev& d, hipStream_t st);
+// wave energy conversion sums (one partial per block and member in
+// diag_part, 3 values, see k_conv)
+hipError_t launch_conv(const Dev& d, hipStream_t st, int* nblocks);
 // levels (1-based, as the reference's height array) of one spectra launch
 struct SpecLevels {
   static constexpr int MAXL = 32;

@@ -39,6 +39,13 @@ struct qg_ctx {
   int evA_live = 1;
   int generation = 0;  // bumped whenever captured launches would change
   double2* stage = nullptr;  // contiguous staging for host transfers (one field)
+  // kept B tendencies (qg_keep_tendencies): buffers allocated on first use,
+  // attached to d.Z / d.tR / d.tI while keep_tend is set; tend_valid once a
+  // step has filled them since they were attached or the state was set
+  int keep_tend = 0;
+  int tend_valid = 0;
+  double2* kz[2] = {nullptr, nullptr};
+  double2* kt[2] = {nullptr, nullptr};
 };
 
 namespace qgi {

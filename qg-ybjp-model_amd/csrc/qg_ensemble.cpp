@@ -303,6 +303,7 @@ int qg_tune(qg_ctx* c, int32_t steps, int32_t* best, double* best_ms) {
   }
   d.tune = cur;
   c->generation++;
+  c->tend_valid = 0;  // kept tendencies are those of a trial step
   hipError_t rr = restore();
   if (rr == hipSuccess) rr = hipStreamSynchronize(c->stream);
   hipFree(snap);

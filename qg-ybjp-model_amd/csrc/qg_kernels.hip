@@ -459,6 +459,82 @@ __global__ __launch_bounds__(NT) void k_diag(Dev d) {
   if (threadIdx.x < 6) d.diag_part[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * 6 + threadIdx.x] = red[threadIdx.x][0];
 }
 
+// Kept tendencies: with Z0, Z1 already y-transformed in place (r2c of
+// zeta BR, zeta BI), the update's combined tendencies tR = nBR - rBI/2 and
+// tI = nBI + rBR/2 are split into the reference's separate terms
+// (convol_waqg derivatives.f90:783-784, refraction_waqg :877-878), all masked
+// by L and zero elsewhere: tR <- nBR, tI <- nBI, Z0 <- rBR, Z1 <- rBI.
+// Under F_LINEAR nB is set to zero as the reference sets it
+// (main_waqg.f90:342-346), not left as the rounding residue of tR + rBI/2.
+__global__ __launch_bounds__(NT) void k_tend_split(Dev d) {
+  const Geo& g = d.g;
+  const size_t n = (size_t)g.nz * g.ikty * g.iktx;
+  const size_t e = (size_t)blockIdx.x * NT + threadIdx.x;
+  if (e >= n) return;
+  const int kx = (int)(e % g.iktx);
+  const size_t row = e / g.iktx;  // z * ikty + iky
+  const int iky = (int)(row % g.ikty);
+  const size_t idx = row * g.sp + kx;
+  double2 rr = zc(), ri = zc(), nr = zc(), ni = zc();
+  if (lmask(kx, iky, g)) {
+    rr = d.Z[0][idx];
+    ri = d.Z[1][idx];
+    if (!(d.flags & F_LINEAR)) {
+      const double2 tr = d.tR[idx], ti = d.tI[idx];
+      nr = make_double2(tr.x + 0.5 * ri.x, tr.y + 0.5 * ri.y);
+      ni = make_double2(ti.x - 0.5 * rr.x, ti.y - 0.5 * rr.y);
+    }
+  }
+  d.Z[0][idx] = rr;
+  d.Z[1][idx] = ri;
+  d.tR[idx] = nr;
+  d.tI[idx] = ni;
+}
+
+// Wave energy conversion sums of we_conversion (diagnostics.f90:967-1076)
+// and gamma_conversion (:1080-1256) by Parseval instead of physical-space
+// products: with D = -kh2 A and every field an L-masked half spectrum,
+//   sum_x f g = n1 n2 sum_k 2 Re(f_k conj(g_k))   (kh2 > 0 modes, which
+// carry their kx = 0 mirror images), so per member, still to be divided by n3:
+//   S0 = sum 2 Re(DR nBR* + DI nBI*)          (advection)
+//   S1 = sum 2 Re(-DR rBI* + DI rBR*)         (refraction)
+//   S2 = sum 2 Re(DR dR* + DI dI*), d = -(c/2) B with c = fw / dt  (dissipation)
+// one partial per block in diag_part (slots 0..2 of 6).
+__global__ __launch_bounds__(NT) void k_conv(Dev d) {
+  const Geo& g = d.g;
+  const int per = g.KX * g.nbox;
+  const int e = blockIdx.x * NT + threadIdx.x;
+  double acc[3] = {0, 0, 0};
+  if (e < per * g.n3) {
+    const int z = e / per, r = e % per, kx = r % g.KX, iky = box_row(r / g.KX, g);
+    const int ky = ky_of(iky, g.n2);
+    const double kh2 = (double)(kx * kx + ky * ky);
+    if (lmask(kx, iky, g) && kh2 > 0.0) {
+      const size_t idx = ((size_t)blockIdx.y * g.n3 + z) * g.ikty * g.sp + (size_t)iky * g.sp + kx;
+      const double2 ar = d.AR[idx], ai = d.AI[idx];
+      const double2 DR = make_double2(-kh2 * ar.x, -kh2 * ar.y), DI = make_double2(-kh2 * ai.x, -kh2 * ai.y);
+      const double2 nR = d.tR[idx], nI = d.tI[idx], rR = d.Z[0][idx], rI = d.Z[1][idx];
+      const double2 br = d.BR[idx], bi = d.BI[idx];
+      const double hc = -0.5 * d.cw[(size_t)iky * g.iktx + kx];
+      acc[0] = 2.0 * ((DR.x * nR.x + DR.y * nR.y) + (DI.x * nI.x + DI.y * nI.y));
+      acc[1] = 2.0 * ((DI.x * rR.x + DI.y * rR.y) - (DR.x * rI.x + DR.y * rI.y));
+      acc[2] = 2.0 * hc * ((DR.x * br.x + DR.y * br.y) + (DI.x * bi.x + DI.y * bi.y));
+    }
+  }
+  __shared__ double red[3][NT];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) red[j][threadIdx.x] = acc[j];
+  __syncthreads();
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+#pragma unroll
+      for (int j = 0; j < 3; ++j) red[j][threadIdx.x] += red[j][threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) d.diag_part[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * 6 + threadIdx.x] = red[threadIdx.x][0];
+}
+
 // ---------------------------------------------------------------------------
 // Regular YBJ operator (ybj_plus = 0, main_waqg.f90:449-461), per (kx, ky)
 // column of a member:
@@ -822,9 +898,9 @@ __global__ __launch_bounds__(NT) void k_row_r2c(Geo g, const double2* __restrict
 }
 
 // forward DFT in y of half rows -> spectral field (all rows, all kx)
+// (in may equal F: every block reads its columns into LDS before it writes)
 template <int N, int TX>
-__global__ __launch_bounds__(NT) void k_col_fwd(Geo g, const double2* __restrict__ tw, const double2* __restrict__ in,
-                                                 double2* __restrict__ F) {
+__global__ __launch_bounds__(NT) void k_col_fwd(Geo g, const double2* __restrict__ tw, const double2* in, double2* F) {
   __shared__ double2 lds[N * TX];
   const int kx0 = blockIdx.x * TX, z = blockIdx.y;
   const double2* iz = in + (size_t)z * g.n2 * g.sp;
@@ -882,6 +958,12 @@ struct L {
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_col_fwd<N, C::TXI>), dim3((d.g.iktx + C::TXI - 1) / C::TXI, nz), dim3(NT), 0, st, d.g,
                        d.tw, d.qs, k);
+    return hipGetLastError();
+  }
+  // forward y transform of nz (kx, y) planes in place, all kx < iktx
+  static hipError_t yfwd(const Dev& d, hipStream_t st, double2* P) {
+    hipLaunchKernelGGL((k_col_fwd<N, C::TXI>), dim3((d.g.iktx + C::TXI - 1) / C::TXI, d.g.nz), dim3(NT), 0, st, d.g,
+                       d.tw, P, P);
     return hipGetLastError();
   }
   static hipError_t c2r(const Dev& d, hipStream_t st, const double2* k, double* r, int ld, int nz) {
@@ -987,7 +1069,33 @@ hipError_t launch_cz(const Dev& d, hipStream_t st, const double2* A, double2* C)
   hipLaunchKernelGGL(k_cz, dim3((unsigned)((n + NT - 1) / NT)), dim3(NT), 0, st, d, A, C);
   return hipGetLastError();
 }
+hipError_t launch_tend_split(const Dev& d, hipStream_t st) {
+  if (!d.Z[0] || !d.Z[1] || !d.tR || !d.tI) return hipErrorInvalidValue;
+  for (int i = 0; i < 2; ++i) {
+    hipError_t e = hipSuccess;
+    switch (d.g.n2) {
+      case 16: e = L<16>::yfwd(d, st, d.Z[i]); break;
+      case 32: e = L<32>::yfwd(d, st, d.Z[i]); break;
+      case 64: e = L<64>::yfwd(d, st, d.Z[i]); break;
+      case 128: e = L<128>::yfwd(d, st, d.Z[i]); break;
+      case 256: e = L<256>::yfwd(d, st, d.Z[i]); break;
+      case 512: e = L<512>::yfwd(d, st, d.Z[i]); break;
+      default: return hipErrorInvalidValue;
+    }
+    if (e != hipSuccess) return e;
+  }
+  const size_t n = (size_t)d.g.nz * d.g.ikty * d.g.iktx;
+  hipLaunchKernelGGL(k_tend_split, dim3((unsigned)((n + NT - 1) / NT)), dim3(NT), 0, st, d);
+  return hipGetLastError();
+}
 int diag_blocks(const Geo& g) { return (g.KX * g.nbox * g.n3 + NT - 1) / NT; }
+hipError_t launch_conv(const Dev& d, hipStream_t st, int* nblocks) {
+  if (!d.Z[0] || !d.Z[1] || !d.tR || !d.tI || !d.cw) return hipErrorInvalidValue;
+  const int nb = diag_blocks(d.g);
+  *nblocks = nb;
+  hipLaunchKernelGGL(k_conv, dim3(nb, d.g.nm), dim3(NT), 0, st, d);
+  return hipGetLastError();
+}
 hipError_t launch_diag(const Dev& d, hipStream_t st, int* nblocks) {
   const int nb = diag_blocks(d.g);
   *nblocks = nb;

@@ -100,8 +100,10 @@ __global__ __launch_bounds__((SH<N, E>::COLT)) void k_ps_inv(Geo g, const double
 //   I4 = i kx F(uBI) + F(zeta BR)/2   I5 = F(vBI)
 // so that after the y transform  n_q = I0 + i ky I1,
 // n_BR - r_BI/2 = I2 + i ky I3 and n_BI + r_BR/2 = I4 + i ky I5.
+// CAP (kept tendencies) also writes F(zeta BR), F(zeta BI) to Z0, Z1, whose
+// y transforms are r_BR, r_BI (refraction_waqg derivatives.f90:877-878).
 // ---------------------------------------------------------------------------
-template <int N, int E, int RP, int SB>
+template <int N, int E, int RP, int SB, bool CAP = false>
 __global__ __launch_bounds__(RP*(N / E)) void k_row_nl(Dev d, int Win, double scale) {
   constexpr int T = N / E, RS = row_xstride<N, E, SB>();
   constexpr int MK0 = ((N / 3 + 2) + T - 1) / T;
@@ -177,6 +179,10 @@ __global__ __launch_bounds__(RP*(N / E)) void k_row_nl(Dev d, int Win, double sc
       keepZR[m] = pas ? zero2() : FzR;
       st_nt(&I[2][ro + k], cscale(w2, scale));
       st_nt(&I[3][ro + k], cscale(w3, scale));
+      if constexpr (CAP) {
+        st_nt(&d.Z[0][ro + k], pas ? zero2() : cscale(FzR, scale));
+        st_nt(&d.Z[1][ro + k], pas ? zero2() : cscale(FzI, scale));
+      }
     }
   }
   // second pair: I4 = i k F(u BI) + F(zeta BR)/2, I5 = F(v BI),
@@ -454,11 +460,12 @@ struct LS {
     hipLaunchKernelGGL((k_ps_inv<N, E>), grid, dim3(S::COLT), 0, st, d.g, d.tw, jobs);
     return hipGetLastError();
   }
-  template <int E, int WG>
+  template <int E, int WG, bool CAP = false>
   static hipError_t row_nl_e(const Dev& d, hipStream_t st, int Win) {
     constexpr int RP = SH<N, E>::template rp<WG>();
     const double scale = 1.0 / ((double)d.g.n1 * (double)d.g.n2);
-    hipLaunchKernelGGL((k_row_nl<N, E, RP, 2>), dim3(d.g.n2 * d.g.nz / RP), dim3(RP * (N / E)), 0, st, d, Win, scale);
+    hipLaunchKernelGGL((k_row_nl<N, E, RP, 2, CAP>), dim3(d.g.n2 * d.g.nz / RP), dim3(RP * (N / E)), 0, st, d, Win,
+                       scale);
     return hipGetLastError();
   }
   template <int E, int SB>
@@ -488,6 +495,7 @@ struct LS {
   // row variants: 0 = E8 in 64-thread blocks, 1 = E16/64, 2 = E8/128,
   // 3 = E16/32 (rows per block = threads / (N/E), at least one)
   static hipError_t row_nl(const Dev& d, hipStream_t st, int Win) {
+    if (d.Z[0]) return row_nl_e<8, 64, true>(d, st, Win);  // kept tendencies: one shape
     switch (d.tune.e16[1]) {
       case 1: return row_nl_e<16, 64>(d, st, Win);
       case 2: return row_nl_e<8, 128>(d, st, Win);

@@ -25,7 +25,8 @@ program qg_driver
   implicit none
 
   integer :: iktx, ikty, ix, iy, iz, it, nxt, u, ios, dump
-  integer :: hunit(10)
+  integer :: hunit(10), cunit, gunit
+  logical :: conv
   integer(c_int) :: rc
   integer(c_int32_t) :: iter
   character(len=512) :: arg
@@ -121,6 +122,12 @@ program qg_driver
   if (hspec_every > 0) call open_spectra()
   if (diag_every > 0) call report(0)
   if (hspec_every > 0) call spectra()
+  if (conv_every > 0) then
+    open(newunit=cunit, file=trim(out_prefix)//'conv.dat', action='write', status='replace')
+    ! before any step the reference's tendency arrays are zero (main_waqg.f90:190)
+    write(cunit, fmt=*) 0.0, 0.0, 0.0
+  end if
+  if (gamma_every > 0) open(newunit=gunit, file=trim(out_prefix)//'gamma.dat', action='write', status='replace')
   if (out_every > 0) call dump_n2()
   if (out_every > 0) call dump_fields(0)
 
@@ -132,7 +139,21 @@ program qg_driver
     if (diag_every > 0) nxt = min(nxt, (it/diag_every + 1)*diag_every)
     if (hspec_every > 0) nxt = min(nxt, (it/hspec_every + 1)*hspec_every)
     if (out_every > 0) nxt = min(nxt, (it/out_every + 1)*out_every)
-    call check(qg_step(ctx, int(nxt - it, c_int32_t)), 'step')
+    if (conv_every > 0) nxt = min(nxt, (it/conv_every + 1)*conv_every)
+    if (gamma_every > 0) nxt = min(nxt, (it/gamma_every + 1)*gamma_every)
+    conv = .false.
+    if (conv_every > 0) conv = mod(nxt, conv_every) == 0
+    if (gamma_every > 0) conv = conv .or. mod(nxt, gamma_every) == 0
+    if (conv) then
+      ! the conversion terms read the last step's tendencies: keep them for that step only
+      if (nxt - it > 1) call check(qg_step(ctx, int(nxt - it - 1, c_int32_t)), 'step')
+      call check(qg_keep_tendencies(ctx, 1_c_int32_t), 'keep_tendencies')
+      call check(qg_step(ctx, 1_c_int32_t), 'step')
+      call conversion(nxt)
+      call check(qg_keep_tendencies(ctx, 0_c_int32_t), 'keep_tendencies')
+    else
+      call check(qg_step(ctx, int(nxt - it, c_int32_t)), 'step')
+    end if
     it = nxt
     if (diag_every > 0) then
       if (mod(it, diag_every) == 0) call report(it)
@@ -149,6 +170,8 @@ program qg_driver
       close(hunit(iz))
     end do
   end if
+  if (conv_every > 0) close(cunit)
+  if (gamma_every > 0) close(gunit)
 
   if (len_trim(out_prefix) > 0) then
     allocate(fk(iktx, ikty, n3, 9))
@@ -263,6 +286,30 @@ contains
       stop 1
     end if
   end subroutine nc_check
+
+  ! conv.dat: time, ca, cr of we_conversion times Uw_scale^2 (diagnostics.f90:1071-1074);
+  ! gamma.dat: time in days, ca, cr, cd of gamma_conversion times Uw_scale^2 U_scale / L_scale
+  ! (:1249-1253); REAL values as the reference writes them
+  subroutine conversion(i)
+    integer, intent(in) :: i
+    real(c_double) :: cv(5)
+    real :: t
+    call check(qg_wave_conversion(ctx, cv), 'wave_conversion')
+    t = real(i*delt)
+    if (conv_every > 0) then
+      if (mod(i, conv_every) == 0) then
+        write(cunit, fmt=*) t, real(cv(1)*Uw_scale*Uw_scale), real(cv(2)*Uw_scale*Uw_scale)
+        flush(cunit)
+      end if
+    end if
+    if (gamma_every > 0) then
+      if (mod(i, gamma_every) == 0) then
+        write(gunit, fmt=*) t*(L_scale/U_scale)/(3600*24), &
+          real(cv(3:5)*Uw_scale*Uw_scale*U_scale/L_scale)
+        flush(gunit)
+      end if
+    end if
+  end subroutine conversion
 
   ! h0w.dat .. h9w.dat (files.f90:182-191)
   subroutine open_spectra()

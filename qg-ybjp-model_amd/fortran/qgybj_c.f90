@@ -86,6 +86,20 @@ module qgybj_c
       real(c_double), intent(out) :: out(*)
     end function
 
+    ! keep each step's wave tendencies nB, rB (on /= 0) for the conversion terms
+    integer(c_int) function qg_keep_tendencies(ctx, on) bind(C, name="qg_keep_tendencies")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: on
+    end function
+
+    ! we_conversion ca, cr and gamma_conversion ca, cr, cd per member, nondimensional
+    integer(c_int) function qg_wave_conversion(ctx, out) bind(C, name="qg_wave_conversion")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: out(*)
+    end function
+
     ! QG vertical velocity w of the current psi (omega equation), shaped like a state field
     integer(c_int) function qg_omega(ctx, wk) bind(C, name="qg_omega")
       import :: c_int, c_ptr

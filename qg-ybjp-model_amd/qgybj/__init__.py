@@ -28,7 +28,8 @@ KERNEL_NAMES = ["col_inv", "row_nl", "col_upd", "row_qw", "col_qw", "solve", "so
 # exported symbols declared in include/qgybj.h
 SYMBOLS = [
     "qg_create", "qg_create_batch", "qg_members", "qg_destroy", "qg_last_error", "qg_set_state", "qg_set_state_physical", "qg_get_state",
-    "qg_step", "qg_iter", "qg_diag", "qg_hspec_waves", "qg_omega", "qg_init_q", "qg_fft_r2c", "qg_fft_c2r", "qg_device_field", "qg_device_pitch", "qg_stream",
+    "qg_step", "qg_iter", "qg_diag", "qg_hspec_waves", "qg_omega", "qg_keep_tendencies",
+    "qg_get_tendencies", "qg_wave_conversion", "qg_init_q", "qg_fft_r2c", "qg_fft_c2r", "qg_device_field", "qg_device_pitch", "qg_stream",
     "qg_sync", "qg_timing", "qg_timing_sample", "qg_kernel_times", "qg_kernel_name", "qg_set_variants", "qg_get_variants",
     "qg_ensemble_create", "qg_ensemble_destroy", "qg_ensemble_error", "qg_ensemble_set_graph_steps",
     "qg_ensemble_step", "qg_tune", "qg_adj_create", "qg_adj_destroy", "qg_adj_error", "qg_adj_set_fused", "qg_adj_objective",
@@ -79,6 +80,9 @@ def load(path: str = LIB_PATH):
     lib.qg_diag.argtypes = [vp, dp]
     lib.qg_hspec_waves.argtypes = [vp, C.POINTER(C.c_int32), C.c_int32, dp]
     lib.qg_omega.argtypes = [vp, dp]
+    lib.qg_keep_tendencies.argtypes = [vp, C.c_int32]
+    lib.qg_get_tendencies.argtypes = [vp, dp, dp, dp, dp]
+    lib.qg_wave_conversion.argtypes = [vp, dp]
     lib.qg_init_q.argtypes = [vp, dp, dp]
     lib.qg_fft_r2c.argtypes = [vp, dp, C.c_int32, dp, C.c_int32]
     lib.qg_fft_c2r.argtypes = [vp, dp, dp, C.c_int32, C.c_int32]
@@ -240,6 +244,25 @@ class Member:
         w = np.empty(self.shape, np.complex128)
         self._chk(self.lib.qg_omega(self._h, _dp(w)), "qg_omega")
         return w
+
+    def keep_tendencies(self, on: bool = True) -> None:
+        """Keep each step's wave tendencies nB, rB (for tendencies() and
+        wave_conversion())."""
+        self._chk(self.lib.qg_keep_tendencies(self._h, 1 if on else 0), "qg_keep_tendencies")
+
+    def tendencies(self):
+        """(nBRk, nBIk, rBRk, rBIk) of the last step (convol_waqg /
+        refraction_waqg), shaped like state fields."""
+        out = [np.empty(self.shape, np.complex128) for _ in range(4)]
+        self._chk(self.lib.qg_get_tendencies(self._h, *[_dp(a) for a in out]), "qg_get_tendencies")
+        return tuple(out)
+
+    def wave_conversion(self) -> np.ndarray:
+        """[we ca, we cr, gamma ca, gamma cr, gamma cd] (diagnostics.f90:967-1256)
+        without the dimensional factors (per member: shape (members, 5))."""
+        out = np.zeros((self.members, 5))
+        self._chk(self.lib.qg_wave_conversion(self._h, _dp(out)), "qg_wave_conversion")
+        return out[0] if self.members == 1 else out
 
     def init_q(self, psik):
         psik = self._c(psik)

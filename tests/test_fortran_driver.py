@@ -228,3 +228,45 @@ def test_driver_netcdf_io(tmp_path):
             # written as default REAL, like the reference (deep levels of surface-trapped waves underflow)
             w32 = want[rows, :2].astype(np.float32).astype(np.float64)
             np.testing.assert_allclose(got[:, 1:3], w32, rtol=2e-6, atol=1e-6 * np.abs(w32).max() + 1e-44)
+
+
+@pytest.mark.gpu
+def test_driver_conversion_files(tmp_path):
+    """conv.dat (we_conversion, diagnostics.f90:1071-1074) at step 0 (zero
+    tendencies, main_waqg.f90:190) and every conv_every steps, gamma.dat
+    (gamma_conversion, :1249-1253) every gamma_every steps, against the
+    oracle's conversion terms with the dimensional factors, written as REAL;
+    keeping the tendencies for those steps leaves the run on the oracle's
+    trajectory."""
+    n1, n3, nsteps, dt = 32, 16, 7, 2e-3
+    P = make_params(n1, n3, dt=dt, W2F=0.05)
+    psi, BR, BI = physical_ics(P, k0=3, seed=3)
+    for name, a in (("psi", psi), ("BR", BR), ("BI", BI)):
+        np.ascontiguousarray(a, dtype=np.float64).tofile(os.path.join(tmp_path, f"ic_{name}.in"))
+    print(f"fortran driver conversion files n1={n1} n3={n3}", flush=True)
+    r = _run(tmp_path, n1=n1, n3=n3, delt=dt, W2F=0.05, itermax=nsteps, conv_every=3, gamma_every=2,
+             ic_prefix=os.path.join(tmp_path, "ic_"), out_prefix=os.path.join(tmp_path, "out_"))
+    assert r.returncode == 0, r.stdout + r.stderr
+    g = O.Grid(n1, n1, n3)
+    OP = oracle_params(P)
+    S = O.initial_state(g, OP, psi, BR, BI)
+    conv = {}
+    for it in range(1, nsteps + 1):
+        S = O.step(g, OP, S)
+        conv[it] = O.wave_conversion(g, OP, S)
+    raw = np.fromfile(os.path.join(tmp_path, "out_state.bin"), dtype=np.complex128).reshape(len(FIELDS), n3, n1, -1)
+    compare_state({f: raw[i] for i, f in enumerate(FIELDS)}, S, 1e-11)
+
+    rows = np.array([[float(x) for x in ln.split()] for ln in open(os.path.join(tmp_path, "out_conv.dat"))])
+    assert rows.shape == (3, 3) and np.all(rows[0] == 0)
+    for row, it in zip(rows[1:], (3, 6)):
+        np.testing.assert_allclose(row[0], np.float32(it * dt), rtol=1e-7)  # list-directed REAL
+        want = (conv[it][:2] * UW_SCALE ** 2).astype(np.float32).astype(np.float64)
+        np.testing.assert_allclose(row[1:], want, rtol=2e-6)
+    rows = np.array([[float(x) for x in ln.split()] for ln in open(os.path.join(tmp_path, "out_gamma.dat"))])
+    assert rows.shape == (3, 4)
+    for row, it in zip(rows, (2, 4, 6)):
+        days = float(np.float32(it * dt)) * (L_SCALE / U_SCALE) / (3600 * 24)
+        np.testing.assert_allclose(row[0], days, rtol=1e-12)
+        want = (conv[it][2:] * UW_SCALE ** 2 * U_SCALE / L_SCALE).astype(np.float32).astype(np.float64)
+        np.testing.assert_allclose(row[1:], want, rtol=2e-6)
