@@ -202,6 +202,74 @@ def adjoint_bench(steps: int = 200, every: int = 20, repeats: int = 2):
             "variants": [int(x) for x in variants]}
 
 
+def rows_bench(steps: int = 50, calls: int = 20) -> dict:
+    """The operator modes and diagnostics beside the step, on the 256^2 x 128
+    production member: the regular YBJ step (dt = 1e-8, where its leapfrog is
+    stable), steps with kept tendencies, and one call each of the conversion
+    terms, the wave spectra at ten levels and the omega equation (ms per call,
+    host synchronisation included)."""
+    import numpy as np
+    import qgybj
+    from qgybj import ic
+    from qgybj.params import YBJ_REGULAR, config_plain_production
+
+    P = config_plain_production()
+    psi = ic.eddy_psi(N1, N3, K0, SEED, Bu=P.Bu)
+    BR, BI = ic.wave_b(N1, N3)
+
+    def per_step(m, n):
+        m.sync()
+        t0 = time.perf_counter()
+        m.step(n)  # ends with a stream synchronisation
+        return 1e3 * (time.perf_counter() - t0) / n
+
+    def per_call(fn, n):
+        fn()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            fn()
+        return 1e3 * (time.perf_counter() - t0) / n
+
+    out = {"workload": "256^2x128 FP64, exponential N^2 (ratio 100), eddies peaked at |k|=5 (seed 1), "
+                       "surface-trapped waves, feedback on"}
+    m = qgybj.Member(P)
+    m.set_state_physical(psi, BR, BI)
+    m.step(6)
+    out["ms_per_step"] = round(per_step(m, steps), 4)
+    m.keep_tendencies(True)
+    m.step(1)
+    out["ms_per_step_tendencies_kept"] = round(per_step(m, steps), 4)
+    conv = m.wave_conversion()
+    out["ms_wave_conversion"] = round(per_call(m.wave_conversion, calls), 4)
+    m.keep_tendencies(False)
+    levels = [1, N3 // 8, N3 // 4, 3 * N3 // 8, N3 // 2, 5 * N3 // 8, 3 * N3 // 4, 7 * N3 // 8, N3 - 2, N3]
+    out["ms_hspec_waves_10_levels"] = round(per_call(lambda: m.hspec_waves(levels), calls), 4)
+    out["ms_omega"] = round(per_call(m.omega, max(calls // 2, 1)), 4)
+    m.close()
+
+    r = qgybj.Member(config_plain_production(dt=1e-8, flags=YBJ_REGULAR))
+    r.set_state_physical(psi, BR, BI)
+    r.step(6)
+    out["ms_per_step_regular_ybj"] = round(per_step(r, steps), 4)
+    r.kernel_times()
+    r.timing(True)
+    r.step(steps)
+    r.timing(False)
+    kt = r.kernel_times()
+    ms, cnt = kt["solve_a"]
+    # sumB / sigma / compute_A: reads B and the two combined tendencies, writes
+    # A and the mean-free B, over the box (8 box fields)
+    _, KXB, nlo, nhi = qgybj.box(N1)
+    box = KXB * (nlo + nhi) * N3 * 16
+    avg_s = ms * 1e-3 / max(cnt, 1)
+    out["k_ybj_regular"] = {"avg_us": round(avg_s * 1e6, 2), "launches": cnt, "alg_bytes_per_launch": 8 * box,
+                            "achieved_GBs": round(8 * box / avg_s / 1e9, 1)}
+    diag = r.diag()
+    r.close()
+    out["finite"] = bool(np.all(np.isfinite(diag)) and np.all(np.isfinite(conv)))
+    return out
+
+
 def large_bench(steps: int = 500, every: int = 50, batch_steps: int = 200, batch_every: int = 20):
     """Forward + adjoint at 512^2 x 256 (exponential N^2, eddies peaked at
     |k| = 10, checkpoints every 50 steps) with a central finite-difference
@@ -330,6 +398,8 @@ def main(argv=None):
                     help="comma-separated kernel variants (qg_set_variants), default: the library's")
     ap.add_argument("--large", action="store_true",
                     help="only the large forward+adjoint (512^2x256) and batched-gradient (4 x 256^2x128) runs")
+    ap.add_argument("--rows", action="store_true",
+                    help="only the regular YBJ step, kept-tendency steps and the diagnostics beside the step")
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run three 1 GiB device copies (HBM counter calibration under rocprofv3 --pmc)")
     a = ap.parse_args(argv)
@@ -337,6 +407,9 @@ def main(argv=None):
         return calib_copy()
     if a.large:
         print(json.dumps(large_bench()))
+        return 0
+    if a.rows:
+        print(json.dumps(rows_bench()))
         return 0
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs one member per GPU on a single MI355X; "

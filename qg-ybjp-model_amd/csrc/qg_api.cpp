@@ -654,13 +654,13 @@ int qg_omega(qg_ctx* c, double* wk) {
   Dev& d = c->d;
   const Geo& g = d.g;
   const size_t nsp = field_elems(g), nph = (size_t)g.n1 * g.n2 * g.nz;
-  // work fields of this diagnostic; the transforms' own scratch (q*) holds
-  // nothing between steps
-  void* blk = nullptr;
-  HIPCHK(c, hipMalloc(&blk, 5 * nsp * sizeof(double2) + 4 * nph * sizeof(double)));
+  // work fields of this diagnostic, kept for the next call (nph is even: the
+  // four physical fields take 2 nph complex slots)
+  if (!c->omega_blk && (rc = dalloc(c, &c->omega_blk, 5 * nsp + 2 * nph)) != QG_OK)
+    return fail(c, rc, "no memory for the omega work fields");
   double2* sp[4];
   double* ph[4];
-  double2* base = static_cast<double2*>(blk);
+  double2* base = c->omega_blk;
   for (int i = 0; i < 4; ++i) sp[i] = base + i * nsp;
   double2* w = base + 4 * nsp;
   double* pb = reinterpret_cast<double*>(base + 5 * nsp);
@@ -669,7 +669,6 @@ int qg_omega(qg_ctx* c, double* wk) {
   if (e == hipSuccess) e = qg::launch_omega(d, c->stream, sp, ph, w);
   if (e == hipSuccess) e = d2h_field(g, wk, w, g.nz, c->stream, c->stage);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  hipFree(blk);
   if (e != hipSuccess) return fail_hip(c, e, "qg_omega");
   return QG_OK;
 }
@@ -684,14 +683,15 @@ int qg_hspec_waves(qg_ctx* c, const int32_t* levels, int32_t nlev, double* out) 
   const int nbin = g.n1 / 2, nm = g.nm;
   const size_t per = (size_t)nbin * 3;
   double* dout = nullptr;
-  HIPCHK(c, hipMalloc(&dout, (size_t)nm * qg::SpecLevels::MAXL * per * sizeof(double)));
+  const size_t nout = (size_t)nm * qg::SpecLevels::MAXL * per;
+  HIPCHK(c, hipMalloc(&dout, nout * (1 + qg::hspec_chunks()) * sizeof(double)));
   std::vector<double> h((size_t)nm * qg::SpecLevels::MAXL * per);
   hipError_t e = hipSuccess;
   for (int i0 = 0; i0 < nlev && e == hipSuccess; i0 += qg::SpecLevels::MAXL) {
     qg::SpecLevels L;
     L.n = std::min(nlev - i0, qg::SpecLevels::MAXL);
     for (int i = 0; i < L.n; ++i) L.lev[i] = levels[i0 + i];
-    e = qg::launch_hspec_waves(c->d, c->stream, L, nbin, dout);
+    e = qg::launch_hspec_waves(c->d, c->stream, L, nbin, dout, dout + nout);
     if (e == hipSuccess) e = hipMemcpyAsync(h.data(), dout, (size_t)nm * L.n * per * sizeof(double), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     for (int m = 0; e == hipSuccess && m < nm; ++m)

@@ -154,7 +154,10 @@ struct SpecLevels {
   int lev[MAXL];
 };
 // out[((member * L.n + level) * nbin + shell) * 3 + {KE, PE, modes}]
-hipError_t launch_hspec_waves(const Dev& d, hipStream_t st, const SpecLevels& L, int nbin, double* out);
+// chunks of the box per level: part holds nm * L.n * hspec_chunks() * nbin * 3
+int hspec_chunks();
+hipError_t launch_hspec_waves(const Dev& d, hipStream_t st, const SpecLevels& L, int nbin, double* out,
+                              double* part);
 hipError_t launch_r2c(const Dev& d, hipStream_t st, const double* r, int ld, double2* k, int nz);
 hipError_t launch_c2r(const Dev& d, hipStream_t st, const double2* k, double* r, int ld, int nz);
 int diag_blocks(const Geo& g);

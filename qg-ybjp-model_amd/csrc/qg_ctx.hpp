@@ -46,6 +46,8 @@ struct qg_ctx {
   int tend_valid = 0;
   double2* kz[2] = {nullptr, nullptr};
   double2* kt[2] = {nullptr, nullptr};
+  // work fields of qg_omega (5 spectral + 4 physical), allocated on its first call
+  double2* omega_blk = nullptr;
 };
 
 namespace qgi {

@@ -798,15 +798,17 @@ hipError_t launch_omega(const Dev& d, hipStream_t st, double2* const sp[4], doub
 //        with C = A_z at the interfaces below and above the level (only the
 //        one above at the bottom level; C = 0 at the top)
 //   modes = 2 per mode.
-// One workgroup per (level, member).  Each round stages one mode per thread
-// in LDS; thread t then sums the staged modes of shells t and t + NSP in
-// staging order, so the sums do not depend on scheduling.
+// One workgroup per (level, member, chunk of HCH of the box): each round
+// stages one mode per thread in LDS; thread t then sums the staged modes of
+// shells t and t + NSP in staging order into the chunk's partial spectrum,
+// and k_hspec_sum adds the chunks' partials in chunk order, so the sums do
+// not depend on scheduling.
 // ---------------------------------------------------------------------------
-constexpr int NSP = 256;
+constexpr int NSP = 256, HCH = 32;
 
-__global__ __launch_bounds__(NSP) void k_hspec_waves(Dev d, SpecLevels L, int nbin, double* __restrict__ out) {
+__global__ __launch_bounds__(NSP) void k_hspec_waves(Dev d, SpecLevels L, int nbin, double* __restrict__ part) {
   const Geo& g = d.g;
-  const int li = blockIdx.x, m = blockIdx.y, t = threadIdx.x;
+  const int li = blockIdx.x, m = blockIdx.y, ch = blockIdx.z, t = threadIdx.x;
   const int z = L.lev[li] - 1;
   const size_t plane = (size_t)g.ikty * g.sp;
   const size_t mz = (size_t)m * g.n3;
@@ -815,11 +817,12 @@ __global__ __launch_bounds__(NSP) void k_hspec_waves(Dev d, SpecLevels L, int nb
   __shared__ int cb[NSP];
   __shared__ double cke[NSP], cpe[NSP];
   double ke[2] = {0.0, 0.0}, pe[2] = {0.0, 0.0}, nm[2] = {0.0, 0.0};
-  for (int base = 0; base < per; base += NSP) {
+  const int len = (per + HCH - 1) / HCH, beg = ch * len, end = min(per, beg + len);
+  for (int base = beg; base < end; base += NSP) {
     const int r = base + t;
     int bin = -1;
     double k1 = 0.0, p1 = 0.0;
-    if (r < per) {
+    if (r < end) {
       const int kx = r % g.KX, iky = box_row(r / g.KX, g);
       if (lmask(kx, iky, g)) {
         const int ky = ky_of(iky, g.n2);
@@ -864,12 +867,31 @@ __global__ __launch_bounds__(NSP) void k_hspec_waves(Dev d, SpecLevels L, int nb
   for (int h = 0; h < 2; ++h) {
     const int b = t + h * NSP;
     if (b < nbin) {
-      double* o = out + (((size_t)m * L.n + li) * nbin + b) * 3;
-      o[0] = 0.5 * ke[h];
-      o[1] = 0.5 * pe[h];
+      double* o = part + ((((size_t)m * L.n + li) * HCH + ch) * nbin + b) * 3;
+      o[0] = ke[h];
+      o[1] = pe[h];
       o[2] = nm[h];
     }
   }
+}
+
+// out[((m * nlev + li) * nbin + b) * 3 + j]: the chunks' partials summed in
+// chunk order, KE and PE halved (diagnostics.f90:627)
+__global__ __launch_bounds__(NT) void k_hspec_sum(int n, int nbin, const double* __restrict__ part,
+                                                   double* __restrict__ out) {
+  const int e = blockIdx.x * NT + threadIdx.x;  // (m * nlev + li) * nbin + b
+  if (e >= n) return;
+  const int b = e % nbin, ml = e / nbin;
+  double s[3] = {0.0, 0.0, 0.0};
+  for (int ch = 0; ch < HCH; ++ch) {
+    const double* p = part + (((size_t)ml * HCH + ch) * nbin + b) * 3;
+    s[0] += p[0];
+    s[1] += p[1];
+    s[2] += p[2];
+  }
+  out[(size_t)e * 3] = 0.5 * s[0];
+  out[(size_t)e * 3 + 1] = 0.5 * s[1];
+  out[(size_t)e * 3 + 2] = s[2];
 }
 
 // ---------------------------------------------------------------------------
@@ -1103,9 +1125,15 @@ hipError_t launch_diag(const Dev& d, hipStream_t st, int* nblocks) {
   return hipGetLastError();
 }
 
-hipError_t launch_hspec_waves(const Dev& d, hipStream_t st, const SpecLevels& L, int nbin, double* out) {
+int hspec_chunks() { return HCH; }
+hipError_t launch_hspec_waves(const Dev& d, hipStream_t st, const SpecLevels& L, int nbin, double* out,
+                              double* part) {
   if (L.n < 1 || L.n > SpecLevels::MAXL || nbin < 1 || nbin > 2 * NSP) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_hspec_waves, dim3(L.n, d.g.nm), dim3(NSP), 0, st, d, L, nbin, out);
+  hipLaunchKernelGGL(k_hspec_waves, dim3(L.n, d.g.nm, HCH), dim3(NSP), 0, st, d, L, nbin, part);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const int n = d.g.nm * L.n * nbin;
+  hipLaunchKernelGGL(k_hspec_sum, dim3((n + NT - 1) / NT), dim3(NT), 0, st, n, nbin, part, out);
   return hipGetLastError();
 }
 
