@@ -151,6 +151,13 @@ int qg_hspec_waves(qg_ctx* ctx, const int32_t* levels, int32_t nlev, double* out
  * outside the mask.  Nondimensional, as the reference's wak. */
 int qg_omega(qg_ctx* ctx, double* wk);
 
+/* we_vave (diagnostics.f90:750-813): the vertically averaged wave energy
+ * density (1/n3) sum_z 0.5 (BR^2 + BI^2) of the current state at every
+ * physical point, from the c2r of BR and BI; out holds n1 * n2 values per
+ * member, member-major, x fastest (the reference's WE_ave(ix, iy)), in FP64
+ * (the reference sums in REAL(4)) and without the Uw_scale^2 factor. */
+int qg_wave_vave(qg_ctx* ctx, double* out);
+
 /* Keep the wave tendencies of every step (on != 0) or stop keeping them.
  * The reference keeps its tendency arrays nBRk, nBIk (convol_waqg,
  * derivatives.f90:607-803) and rBRk, rBIk (refraction_waqg :810-900) from

@@ -845,6 +845,15 @@ def wave_conversion(grid: Grid, P: Params, S: State) -> np.ndarray:
                      g_ca / (nv * P.Bu), g_cr / (nv * P.Bu), g_cd / (nv * P.Bu)])
 
 
+def we_vave(grid: Grid, S: State) -> np.ndarray:
+    """we_vave (diagnostics.f90:750-813): WE(x, y) = sum_z 0.5 (BR^2 + BI^2)
+    over the c2r fields (:762-778), divided by n3 (:787), in FP64 (the
+    reference accumulates in REAL(4)) and without Uw_scale^2; shape (n2, n1)."""
+    BR = fft_c2r(S.BRk)
+    BI = fft_c2r(S.BIk)
+    return np.sum(0.5 * (BR * BR + BI * BI), axis=0) / grid.n3
+
+
 def rel_l2(a, b, grid: Grid | None = None) -> float:
     """Relative L2 difference ||a-b||/||b|| over L = 1 modes (all modes if
     no grid is given)."""

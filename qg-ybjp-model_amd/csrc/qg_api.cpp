@@ -647,6 +647,26 @@ int qg_wave_conversion(qg_ctx* c, double* out) {
   return QG_OK;
 }
 
+int qg_wave_vave(qg_ctx* c, double* out) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!out) return fail(c, QG_ERR_ARG, "null output");
+  Dev& d = c->d;
+  const Geo& g = d.g;
+  const size_t nsp = field_elems(g), nph = (size_t)g.n1 * g.n2 * g.nz;
+  // the omega work fields: two physical fields for BR, BI and a third for the map
+  if (!c->omega_blk && (rc = dalloc(c, &c->omega_blk, 5 * nsp + 2 * nph)) != QG_OK)
+    return fail(c, rc, "no memory for the diagnostic work fields");
+  double* pb = reinterpret_cast<double*>(c->omega_blk + 5 * nsp);
+  hipError_t e = qg::launch_vave(d, c->stream, pb, pb + nph, pb + 2 * nph);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(out, pb + 2 * nph, (size_t)g.n1 * g.n2 * g.nm * sizeof(double), hipMemcpyDeviceToHost,
+                       c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return fail_hip(c, e, "qg_wave_vave");
+  return QG_OK;
+}
+
 int qg_omega(qg_ctx* c, double* wk) {
   int rc = check_ctx(c);
   if (rc) return rc;

@@ -147,6 +147,9 @@ hipError_t launch_tend_split(const Dev& d, hipStream_t st);
 // wave energy conversion sums (one partial per block and member in
 // diag_part, 3 values, see k_conv)
 hipError_t launch_conv(const Dev& d, hipStream_t st, int* nblocks);
+// we_vave: c2r of BR, BI into br, bi (n1 * n2 * nz each), then the vertical
+// average of 0.5 (BR^2 + BI^2) per (member, y, x) into out
+hipError_t launch_vave(const Dev& d, hipStream_t st, double* br, double* bi, double* out);
 // levels (1-based, as the reference's height array) of one spectra launch
 struct SpecLevels {
   static constexpr int MAXL = 32;

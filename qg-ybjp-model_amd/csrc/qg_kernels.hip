@@ -894,6 +894,24 @@ __global__ __launch_bounds__(NT) void k_hspec_sum(int n, int nbin, const double*
   out[(size_t)e * 3 + 2] = s[2];
 }
 
+// we_vave (diagnostics.f90:750-813): the vertically averaged wave energy
+// density 0.5 (BR^2 + BI^2) / n3 per physical point (x, y) of each member,
+// from the c2r fields br, bi ((z, y, x) order, member-major); out is
+// (member, y, x).
+__global__ __launch_bounds__(NT) void k_vave(Geo g, const double* __restrict__ br, const double* __restrict__ bi,
+                                             double* __restrict__ out) {
+  const size_t nxy = (size_t)g.n1 * g.n2;
+  const size_t e = (size_t)blockIdx.x * NT + threadIdx.x;  // m * nxy + y * n1 + x
+  if (e >= nxy * g.nm) return;
+  const size_t m = e / nxy, xy = e % nxy;
+  double s = 0.0;
+  for (int z = 0; z < g.n3; ++z) {
+    const size_t i = ((size_t)m * g.n3 + z) * nxy + xy;
+    s += 0.5 * (br[i] * br[i] + bi[i] * bi[i]);
+  }
+  out[e] = s / g.n3;
+}
+
 // ---------------------------------------------------------------------------
 // raw transforms (fft.f90:55-83) for initial conditions and tests
 // ---------------------------------------------------------------------------
@@ -1009,6 +1027,14 @@ struct L {
 
 bool supported_n(int n) { return n == 16 || n == 32 || n == 64 || n == 128 || n == 256 || n == 512; }
 
+hipError_t launch_vave(const Dev& d, hipStream_t st, double* br, double* bi, double* out) {
+  hipError_t e = launch_c2r(d, st, d.BR, br, d.g.n1, d.g.nz);
+  if (e == hipSuccess) e = launch_c2r(d, st, d.BI, bi, d.g.n1, d.g.nz);
+  if (e != hipSuccess) return e;
+  const size_t n = (size_t)d.g.n1 * d.g.n2 * d.g.nm;
+  hipLaunchKernelGGL(k_vave, dim3((unsigned)((n + NT - 1) / NT)), dim3(NT), 0, st, d.g, br, bi, out);
+  return hipGetLastError();
+}
 hipError_t launch_r2c(const Dev& d, hipStream_t st, const double* r, int ld, double2* k, int nz) {
   QG_DISPATCH(d.g.n1, r2c(d, st, r, ld, k, nz))
 }

@@ -25,7 +25,7 @@ program qg_driver
   implicit none
 
   integer :: iktx, ikty, ix, iy, iz, it, nxt, u, ios, dump
-  integer :: hunit(10), cunit, gunit
+  integer :: hunit(10), cunit, gunit, nvave
   logical :: conv
   integer(c_int) :: rc
   integer(c_int32_t) :: iter
@@ -128,6 +128,8 @@ program qg_driver
     write(cunit, fmt=*) 0.0, 0.0, 0.0
   end if
   if (gamma_every > 0) open(newunit=gunit, file=trim(out_prefix)//'gamma.dat', action='write', status='replace')
+  nvave = 0
+  if (vave_every > 0) call vave()
   if (out_every > 0) call dump_n2()
   if (out_every > 0) call dump_fields(0)
 
@@ -141,6 +143,7 @@ program qg_driver
     if (out_every > 0) nxt = min(nxt, (it/out_every + 1)*out_every)
     if (conv_every > 0) nxt = min(nxt, (it/conv_every + 1)*conv_every)
     if (gamma_every > 0) nxt = min(nxt, (it/gamma_every + 1)*gamma_every)
+    if (vave_every > 0) nxt = min(nxt, (it/vave_every + 1)*vave_every)
     conv = .false.
     if (conv_every > 0) conv = mod(nxt, conv_every) == 0
     if (gamma_every > 0) conv = conv .or. mod(nxt, gamma_every) == 0
@@ -163,6 +166,9 @@ program qg_driver
     end if
     if (hspec_every > 0) then
       if (mod(it, hspec_every) == 0) call spectra()
+    end if
+    if (vave_every > 0) then
+      if (mod(it, vave_every) == 0) call vave()
     end if
   end do
   if (hspec_every > 0) then
@@ -310,6 +316,26 @@ contains
       end if
     end if
   end subroutine conversion
+
+  ! WE_vave<count>.dat (diagnostics.f90:780-798): the map times Uw_scale^2, one value per line
+  ! in E12.5 for x = 1 .. n1, then a blank line, for each y; the name as the reference's
+  ! (A7,I3,A4) forms it
+  subroutine vave()
+    real(c_double), allocatable :: we(:, :)
+    character(len=14) :: fname
+    integer :: v, jx, jy
+    allocate(we(n1, n1))
+    call check(qg_wave_vave(ctx, we), 'wave_vave')
+    write(fname, '(A7,I3,A4)') 'WE_vave', nvave, '.dat'
+    open(newunit=v, file=trim(out_prefix)//fname, action='write', status='replace')
+    do jy = 1, n1
+      write(v, '(1x,E12.5,1x)') (real(we(jx, jy)*Uw_scale*Uw_scale), jx = 1, n1)
+      write(v, fmt=*) '           '
+    end do
+    close(v)
+    nvave = nvave + 1
+    deallocate(we)
+  end subroutine vave
 
   ! h0w.dat .. h9w.dat (files.f90:182-191)
   subroutine open_spectra()

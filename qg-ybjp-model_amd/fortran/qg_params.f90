@@ -51,6 +51,9 @@ module qg_run_params
   ! at step 0 and every conv_every steps) and <out_prefix>gamma.dat (gamma_conversion,
   ! :1080-1256; every gamma_every steps), as the reference writes them (files.f90:140-141)
   integer :: conv_every = 0, gamma_every = 0
+  ! vertically averaged wave energy density maps (we_vave, diagnostics.f90:750-813) to
+  ! <out_prefix>WE_vave<n>.dat at step 0 and every vave_every steps (0: never)
+  integer :: vave_every = 0
   integer :: init_ncf_psi = 0, init_ncf_la = 0
   character(len=256) :: init_ncf_psi_filename = 'psi000.in.nc', init_ncf_la_filename = 'la000.in.nc'
   double precision :: U_scale = 0.25, Uw_scale = 2.5e-5
@@ -62,7 +65,7 @@ module qg_run_params
                 no_dispersion, inviscid, ybj_plus, stratification, n2_ratio, itermax, diag_every, out_every, &
                 ic_prefix, out_prefix, grad_steps, grad_every, grad_zdeep, init_ncf_psi, init_ncf_la, &
                 init_ncf_psi_filename, init_ncf_la_filename, U_scale, Uw_scale, L_scale, N0, hspec_every, &
-                conv_every, gamma_every
+                conv_every, gamma_every, vave_every
 
 contains
 

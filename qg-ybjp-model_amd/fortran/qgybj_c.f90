@@ -86,6 +86,13 @@ module qgybj_c
       real(c_double), intent(out) :: out(*)
     end function
 
+    ! vertically averaged wave energy density 0.5 (BR^2 + BI^2) / n3, out(n1, n2) per member
+    integer(c_int) function qg_wave_vave(ctx, out) bind(C, name="qg_wave_vave")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: out(*)
+    end function
+
     ! keep each step's wave tendencies nB, rB (on /= 0) for the conversion terms
     integer(c_int) function qg_keep_tendencies(ctx, on) bind(C, name="qg_keep_tendencies")
       import :: c_int, c_int32_t, c_ptr

@@ -28,7 +28,7 @@ KERNEL_NAMES = ["col_inv", "row_nl", "col_upd", "row_qw", "col_qw", "solve", "so
 # exported symbols declared in include/qgybj.h
 SYMBOLS = [
     "qg_create", "qg_create_batch", "qg_members", "qg_destroy", "qg_last_error", "qg_set_state", "qg_set_state_physical", "qg_get_state",
-    "qg_step", "qg_iter", "qg_diag", "qg_hspec_waves", "qg_omega", "qg_keep_tendencies",
+    "qg_step", "qg_iter", "qg_diag", "qg_hspec_waves", "qg_omega", "qg_wave_vave", "qg_keep_tendencies",
     "qg_get_tendencies", "qg_wave_conversion", "qg_init_q", "qg_fft_r2c", "qg_fft_c2r", "qg_device_field", "qg_device_pitch", "qg_stream",
     "qg_sync", "qg_timing", "qg_timing_sample", "qg_kernel_times", "qg_kernel_name", "qg_set_variants", "qg_get_variants",
     "qg_ensemble_create", "qg_ensemble_destroy", "qg_ensemble_error", "qg_ensemble_set_graph_steps",
@@ -80,6 +80,7 @@ def load(path: str = LIB_PATH):
     lib.qg_diag.argtypes = [vp, dp]
     lib.qg_hspec_waves.argtypes = [vp, C.POINTER(C.c_int32), C.c_int32, dp]
     lib.qg_omega.argtypes = [vp, dp]
+    lib.qg_wave_vave.argtypes = [vp, dp]
     lib.qg_keep_tendencies.argtypes = [vp, C.c_int32]
     lib.qg_get_tendencies.argtypes = [vp, dp, dp, dp, dp]
     lib.qg_wave_conversion.argtypes = [vp, dp]
@@ -244,6 +245,15 @@ class Member:
         w = np.empty(self.shape, np.complex128)
         self._chk(self.lib.qg_omega(self._h, _dp(w)), "qg_omega")
         return w
+
+    def wave_vave(self) -> np.ndarray:
+        """Vertically averaged wave energy density 0.5 (BR^2 + BI^2) / n3 per
+        physical point (we_vave, diagnostics.f90:750-813), shape (n2, n1)
+        (a leading members axis for a batch), without Uw_scale^2."""
+        lead = () if self.members == 1 else (self.members,)
+        out = np.zeros(lead + (self.n1, self.n1))
+        self._chk(self.lib.qg_wave_vave(self._h, _dp(out)), "qg_wave_vave")
+        return out
 
     def keep_tendencies(self, on: bool = True) -> None:
         """Keep each step's wave tendencies nB, rB (for tendencies() and

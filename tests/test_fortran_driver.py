@@ -270,3 +270,38 @@ def test_driver_conversion_files(tmp_path):
         np.testing.assert_allclose(row[0], days, rtol=1e-12)
         want = (conv[it][2:] * UW_SCALE ** 2 * U_SCALE / L_SCALE).astype(np.float32).astype(np.float64)
         np.testing.assert_allclose(row[1:], want, rtol=2e-6)
+
+
+@pytest.mark.gpu
+def test_driver_we_vave_files(tmp_path):
+    """WE_vave<count>.dat (we_vave, diagnostics.f90:780-798) at step 0 and
+    every vave_every steps: the (A7,I3,A4) file names, n2 blocks of n1 E12.5
+    values, each block followed by a blank line, equal to the oracle's map
+    times Uw_scale^2 to the printed 5 digits."""
+    n1, n3, nsteps, every, dt = 32, 8, 6, 3, 2e-3
+    P = make_params(n1, n3, dt=dt, W2F=0.05)
+    psi, BR, BI = physical_ics(P, k0=3, seed=8)
+    for name, a in (("psi", psi), ("BR", BR), ("BI", BI)):
+        np.ascontiguousarray(a, dtype=np.float64).tofile(os.path.join(tmp_path, f"ic_{name}.in"))
+    print(f"fortran driver we_vave n1={n1} n3={n3}", flush=True)
+    r = _run(tmp_path, n1=n1, n3=n3, delt=dt, W2F=0.05, itermax=nsteps, vave_every=every,
+             ic_prefix=os.path.join(tmp_path, "ic_"), out_prefix=os.path.join(tmp_path, "out_"))
+    assert r.returncode == 0, r.stdout + r.stderr
+    g = O.Grid(n1, n1, n3)
+    OP = oracle_params(P)
+    S = O.initial_state(g, OP, psi, BR, BI)
+    maps = {0: O.we_vave(g, S)}
+    for it in range(1, nsteps + 1):
+        S = O.step(g, OP, S)
+        if it % every == 0:
+            maps[it] = O.we_vave(g, S)
+    for count, it in enumerate(sorted(maps)):
+        lines = open(os.path.join(tmp_path, f"out_WE_vave{count:3d}.dat")).read().splitlines()
+        assert len(lines) == n1 * (n1 + 1)
+        vals = []
+        for y in range(n1):
+            block = lines[y * (n1 + 1):(y + 1) * (n1 + 1)]
+            assert block[-1].strip() == ""
+            vals.append([float(v) for v in block[:-1]])
+        want = maps[it] * UW_SCALE ** 2
+        np.testing.assert_allclose(np.array(vals), want, rtol=1e-4 * 1.0001, atol=0)

@@ -5,6 +5,8 @@
   kinetic and potential parts to 1e-12; deterministic (bitwise equal on
   repeat); batch members equal single members bit for bit; more levels than
   one launch takes; bad levels rejected;
+* the vertically averaged wave energy density (qg_wave_vave, we_vave
+  diagnostics.f90:750-813) to 1e-12, batch members equal single members;
 * the omega equation (qg_omega, derivatives.f90:1469-1586 and
   elliptic.f90:463-567) on a running state to 1e-11; the run continues
   bit-identically; batch members equal single members.
@@ -114,4 +116,35 @@ def test_omega_batch_members():
         one.step(3)
         assert np.array_equal(one.omega(), wb[j])
         one.close()
+    b.close()
+
+
+def test_we_vave_matches_oracle_and_batch():
+    """qg_wave_vave (we_vave, diagnostics.f90:750-813) on a running state
+    against the oracle to 1e-12; batch members equal single members bit for
+    bit; the run continues unchanged."""
+    P = make_params(64, 16, dt=2e-3, W2F=0.05, stratification="exponential")
+    ics = [physical_ics(P, k0=3, seed=s) for s in (5, 6)]
+    g = O.Grid(64, 64, 16)
+    b = qgybj.Member(P, members=2)
+    b.set_state_physical(*[np.stack([x[i] for x in ics]) for i in range(3)])
+    b.step(6)
+    vb = b.wave_vave()
+    assert vb.shape == (2, 64, 64)
+    for j, ic in enumerate(ics):
+        one, twin = qgybj.Member(P), qgybj.Member(P)
+        for m in (one, twin):
+            m.set_state_physical(*ic)
+            m.step(6)
+        v = one.wave_vave()
+        assert np.array_equal(v, vb[j])
+        want = O.we_vave(g, _oracle_state(one.get_state()))
+        np.testing.assert_allclose(v, want, rtol=1e-12, atol=1e-14 * want.max())
+        one.step(3)
+        twin.step(3)
+        s1, s2 = one.get_state(), twin.get_state()
+        for f in s1:
+            assert np.array_equal(s1[f], s2[f]), f
+        one.close()
+        twin.close()
     b.close()

@@ -80,3 +80,22 @@ def test_kernel_shell_index_matches_reference_expression():
     kh = np.sqrt(kh2)
     ref = np.float32(kh * TWOPI / TWOPI + 0.5).astype(np.int64)
     assert np.array_equal(np.floor(kh + 0.5).astype(np.int64), ref)
+
+
+def test_we_vave_single_modes():
+    """we_vave (diagnostics.f90:750-813) of BR holding one mode a at (kx, ky)
+    and BI one mode b at (0, ky'): c2r gives 2 Re(a e^{i(kx x + ky y)}) (the
+    realit mirror supplies the kx = 0 conjugate), so the map is
+    0.5 (f_R^2 + f_I^2) / n3 at every grid point."""
+    g = O.Grid(16, 16, 5)
+    BR = np.zeros((g.n3, g.ikty, g.iktx), np.complex128)
+    BI = np.zeros_like(BR)
+    a, b = 0.3 - 0.4j, -0.2 + 0.7j
+    BR[2, g.ikty - 3, 2] = a  # (kx, ky) = (2, -3)
+    BI[4, 5, 0] = b           # (0, 5)
+    S = _state(g, BRk=BR, BIk=BI)
+    x = TWOPI * np.arange(16) / 16
+    X, Y = np.meshgrid(x, x, indexing="xy")  # (y, x)
+    fR = 2 * (a * np.exp(1j * (2 * X - 3 * Y))).real
+    fI = 2 * (b * np.exp(1j * (5 * Y))).real
+    np.testing.assert_allclose(O.we_vave(g, S), 0.5 * (fR ** 2 + fI ** 2) / g.n3, rtol=1e-13, atol=1e-15)
