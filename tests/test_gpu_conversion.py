@@ -69,10 +69,13 @@ def test_tendencies_and_conversion_match_oracle(flags, dt):
     m.close()
 
 
-def test_tendencies_reproduce_the_update():
+@pytest.mark.parametrize("n1,n3", [(32, 8), (256, 16)])
+def test_tendencies_reproduce_the_update(n1, n3):
     """B^{n+1} = L [B^{n-1} e^{-2fw} - 2 dt (nBR + c kh2 AI^n - rBI/2) e^{-fw}]
-    (main_waqg.f90:376) from the states around one leapfrog step."""
-    P = make_params(32, 8, dt=2e-3, W2F=0.05)
+    (main_waqg.f90:376) from the states around one leapfrog step, and the
+    separated refraction terms equal the oracle's refraction_waqg of the state
+    before the step (1e-11), also at the production width n1 = 256."""
+    P = make_params(n1, n3, dt=2e-3, W2F=0.05)
     psi, BR, BI = physical_ics(P, k0=3, seed=6)
     m = qgybj.Member(P)
     m.set_state_physical(psi, BR, BI)
@@ -82,8 +85,10 @@ def test_tendencies_reproduce_the_update():
     m.step(1)
     after = m.get_state()
     nBR, nBI, rBR, rBI = m.tendencies()
-    g = O.Grid(32, 32, 8)
+    g = O.Grid(n1, n1, n3)
     L = g.Lb
+    wR, wI = O.refraction_waqg(g, before["BRk"], before["BIk"], before["psik"])
+    assert rel_l2(rBR, wR) < 1e-11 and rel_l2(rBI, wI) < 1e-11
     kh2 = g.kh2.astype(np.float64)
     c = 0.5 / (P.Bu * P.Ro)
     e1, e2 = np.exp(-P.ifac_w), np.exp(-2 * P.ifac_w)
