@@ -702,9 +702,10 @@ int qg_hspec_waves(qg_ctx* c, const int32_t* levels, int32_t nlev, double* out) 
     if (levels[i] < 1 || levels[i] > g.n3) return fail(c, QG_ERR_ARG, "spectrum level outside 1 .. n3");
   const int nbin = g.n1 / 2, nm = g.nm;
   const size_t per = (size_t)nbin * 3;
-  double* dout = nullptr;
   const size_t nout = (size_t)nm * qg::SpecLevels::MAXL * per;
-  HIPCHK(c, hipMalloc(&dout, nout * (1 + qg::hspec_chunks()) * sizeof(double)));
+  if (!c->spec_blk && (rc = dalloc(c, &c->spec_blk, nout * (1 + qg::hspec_chunks()))) != QG_OK)
+    return fail(c, rc, "no memory for the spectra");
+  double* dout = c->spec_blk;
   std::vector<double> h((size_t)nm * qg::SpecLevels::MAXL * per);
   hipError_t e = hipSuccess;
   for (int i0 = 0; i0 < nlev && e == hipSuccess; i0 += qg::SpecLevels::MAXL) {
@@ -718,7 +719,6 @@ int qg_hspec_waves(qg_ctx* c, const int32_t* levels, int32_t nlev, double* out) 
       std::copy(h.begin() + (size_t)m * L.n * per, h.begin() + (size_t)(m + 1) * L.n * per,
                 out + ((size_t)m * nlev + i0) * per);
   }
-  hipFree(dout);
   HIPCHK(c, e);
   return QG_OK;
 }

@@ -48,6 +48,8 @@ struct qg_ctx {
   double2* kt[2] = {nullptr, nullptr};
   // work fields of qg_omega (5 spectral + 4 physical), allocated on its first call
   double2* omega_blk = nullptr;
+  // spectra output and per-chunk partials of qg_hspec_waves, allocated on its first call
+  double* spec_blk = nullptr;
 };
 
 namespace qgi {
